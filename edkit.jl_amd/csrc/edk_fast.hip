@@ -1,0 +1,472 @@
+// edk_fast.hip -- specialised gather kernel for translation-invariant
+// two-site operators on 32-bit packed words.
+//
+// Applies when (checked on the host, else the generic kernel runs):
+//   - base 2, L <= 32, so a configuration is one 32-bit word;
+//   - the basis is TranslationParityBasis, TranslationalBasis (any k, unit
+//     cell a) or ProjectedBasis, built by the library (orbit norms known);
+//   - the operator is a sum of "families": for a fixed 2x2-site matrix M and
+//     site distance d, the terms M on [i, i+d mod L] for every i -- exactly
+//     what trans_inv_operator(M, [1, 1+d], B) builds (Operator.jl:102-108);
+//   - M is real.
+//
+// Row i with configuration s: for a family (M, d) let r = rotl(s, d); bit q of
+// (s, r) is the pair (site i, site i+d) of the term whose first site sits at
+// bit q.  The diagonal is sum_ab M[ab,ab] * popcount(sel_a(s) & sel_b(r));
+// every off-diagonal entry M[loc, c] acts at the bits of
+// sel(loc_a, s) & sel(loc_b, r), flipping (loc ^ c) at (q, q - d).  Targets
+// are processed eight at a time: canonicalise all eight, then issue their
+// directory reads, their in-bucket binary searches and their x gathers
+// together, so each lane keeps eight independent memory requests in flight.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "edk_matvec.h"
+
+#define FAST_MAXFAM 4
+#define FAST_MAXENT 12
+#define FAST_BATCH 8
+#define FAST_TPB 256
+
+struct FamEnt {
+  int sel;     // bit 1: first site must be 1, bit 0: second site must be 1
+  int flip;    // bit 1: flip first site, bit 0: flip second site
+  double val;  // M[loc, c]
+};
+
+struct FastFam {
+  int d;
+  int nent;
+  double diag[4];  // M[ab, ab]
+  FamEnt ent[FAST_MAXENT];
+};
+
+struct FastTab {
+  int nfam;
+  FastFam fam[FAST_MAXFAM];
+};
+
+struct FastArgs {
+  int kind, L, A, len, kp, P;
+  uint32_t mask;
+  int ndiv;
+  int divs[16];  // divisors of len, ascending (orbit period candidates)
+  const FastTab *tab;
+  const double *Rtab;  // orbit norms (same table the enumerator used)
+  const double2 *ph;   // TranslationalBasis phases C[m]
+  const uint32_t *keys;
+  const uint32_t *dir;
+  int dshift;
+  uint32_t ndir;
+  int64_t D;
+  const double *Rrow;
+};
+
+__device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
+  if (L == 32) return __builtin_amdgcn_alignbit(s, s, sh);
+  uint32_t hi = s >> (32 - L), lo = s | (s << L);
+  return __builtin_amdgcn_alignbit(hi, lo, sh) & mask;
+}
+
+__device__ __forceinline__ uint32_t rev32(uint32_t s, int L) { return __brev(s) >> (32 - L); }
+
+// Canonicalise s'.  Returns false when the representative cannot be in the
+// table; otherwise rep and the real or complex factor conj(phase) * R_rep.
+template <int KIND, bool CPLX>
+__device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
+                                           double &ci) {
+  if (KIND == EDK_PROJECTED) {
+    rep = s;
+    cr = 1.0;
+    ci = 0.0;
+    return true;
+  }
+  const int sh = A.A;
+  // orbit period: smallest divisor p of len with rot^p(s) == s
+  int period = A.len;
+  for (int q = 0; q < A.ndiv; q++) {
+    int p = A.divs[q];
+    if (p < A.len && rotr32(s, p * sh, A.L, A.mask) == s) {
+      period = p;
+      break;
+    }
+  }
+  uint32_t minA = s, cur = s;
+  for (int i = 1; i < period; i++) {
+    cur = rotr32(cur, sh, A.L, A.mask);
+    minA = min(minA, cur);
+  }
+  if (KIND == EDK_TRANSLATIONAL) {
+    if (((int64_t)period * A.kp) % A.len) return false;  // check_momentum
+    double R = A.Rtab[period - 1];
+    if (CPLX || A.kp != 0) {
+      int M = 0;
+      cur = s;
+      while (cur != minA) {
+        cur = rotr32(cur, sh, A.L, A.mask);
+        M++;
+      }
+      double2 c = A.ph[M];
+      cr = c.x * R;
+      ci = -c.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    rep = minA;
+    return true;
+  }
+  // TranslationParityBasis
+  uint32_t rb = rev32(s, A.L);
+  uint32_t minB = rb;
+  cur = rb;
+  for (int i = 1; i < period; i++) {
+    cur = rotr32(cur, sh, A.L, A.mask);
+    minB = min(minB, cur);
+  }
+  bool qp = (minA == minB);
+  double R = A.Rtab[(qp ? A.len : 0) + period - 1];
+  double sgn = 1.0;
+  if (A.kp != 1 || A.P != 1) {
+    // index of the first minimum along the orbit; the reflected branch uses
+    // reverse(rot^M(s)) == rot^(-M)(reverse(s)) (TranslationalParityBasis.jl:222-231)
+    if (minB < minA) {
+      int M = 0;
+      cur = s;
+      while (rev32(cur, A.L) != minB) {
+        cur = rotr32(cur, sh, A.L, A.mask);
+        M++;
+      }
+      sgn = (double)A.P * ((A.kp == 1 || (M & 1) == 0) ? 1.0 : -1.0);
+    } else {
+      int M = 0;
+      cur = s;
+      while (cur != minA) {
+        cur = rotr32(cur, sh, A.L, A.mask);
+        M++;
+      }
+      sgn = (A.kp == 1 || (M & 1) == 0) ? 1.0 : -1.0;
+    }
+  }
+  rep = minA < minB ? minA : minB;
+  cr = sgn * R;
+  ci = 0.0;
+  return true;
+}
+
+template <int KIND, bool CPLX, typename VT, typename ST>
+__global__ void __launch_bounds__(FAST_TPB) k_matvec_fast(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
+                                                          int accumulate) {
+  int64_t i = (int64_t)blockIdx.x * FAST_TPB + threadIdx.x;
+  if (i >= A.D) return;
+  const FastTab &T = *A.tab;
+  const uint32_t s = A.keys[i];
+  const uint32_t ns = ~s & A.mask;
+  ST dg = 0;
+  ST orr = 0, oi = 0;
+  // target generator state
+  int f = 0, e = 0;
+  uint32_t bits = 0;
+  uint32_t r = 0;
+  bool more = T.nfam > 0;
+  // diagonal by popcounts
+  for (int ff = 0; ff < T.nfam; ff++) {
+    const FastFam &F = T.fam[ff];
+    uint32_t rr = rotr32(s, A.L - F.d, A.L, A.mask);  // rotl by d
+    uint32_t nr = ~rr & A.mask;
+    dg += (ST)(F.diag[0] * __popc(ns & nr) + F.diag[1] * __popc(ns & rr) + F.diag[2] * __popc(s & nr) +
+               F.diag[3] * __popc(s & rr));
+  }
+  if (more) {
+    r = rotr32(s, A.L - T.fam[0].d, A.L, A.mask);
+    bits = 0;
+    e = -1;
+  }
+  while (true) {
+    uint32_t tg[FAST_BATCH];
+    double tv[FAST_BATCH];
+    int n = 0;
+    // collect up to FAST_BATCH targets
+    while (n < FAST_BATCH && more) {
+      if (bits == 0) {
+        // advance to the next entry / family
+        e++;
+        while (f < T.nfam && e >= T.fam[f].nent) {
+          f++;
+          e = 0;
+          if (f < T.nfam) r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+        }
+        if (f >= T.nfam) {
+          more = false;
+          break;
+        }
+        const FamEnt &E = T.fam[f].ent[e];
+        uint32_t a = (E.sel & 2) ? s : ns;
+        uint32_t b = (E.sel & 1) ? r : (~r & A.mask);
+        bits = a & b;
+        continue;
+      }
+      int q = __ffs(bits) - 1;
+      bits &= bits - 1;
+      const FamEnt &E = T.fam[f].ent[e];
+      int q2 = q - T.fam[f].d;
+      if (q2 < 0) q2 += A.L;
+      uint32_t t = s;
+      if (E.flip & 2) t ^= 1u << q;
+      if (E.flip & 1) t ^= 1u << q2;
+      tg[n] = t;
+      tv[n] = E.val;
+      n++;
+    }
+    if (n == 0) break;
+    // canonicalise
+    uint32_t rep[FAST_BATCH];
+    double cr[FAST_BATCH], ci[FAST_BATCH];
+    int64_t lo[FAST_BATCH], hi[FAST_BATCH];
+    bool valid[FAST_BATCH];
+#pragma unroll
+    for (int q = 0; q < FAST_BATCH; q++) {
+      lo[q] = 0;
+      hi[q] = 0;
+      valid[q] = false;
+      if (q < n) {
+        bool ok = fast_canon<KIND, CPLX>(A, tg[q], rep[q], cr[q], ci[q]);
+        if (ok) {
+          valid[q] = true;
+          uint32_t bkt = rep[q] >> A.dshift;
+          if (bkt < A.ndir) {
+            lo[q] = A.dir[bkt];
+            hi[q] = A.dir[bkt + 1];
+          }
+        }
+      }
+    }
+    // in-bucket lower_bound, all targets advancing together
+    bool act = true;
+    while (act) {
+      act = false;
+#pragma unroll
+      for (int q = 0; q < FAST_BATCH; q++) {
+        if (lo[q] < hi[q]) {
+          int64_t mid = (lo[q] + hi[q]) >> 1;
+          if (A.keys[mid] < rep[q]) lo[q] = mid + 1;
+          else hi[q] = mid;
+          act = true;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < FAST_BATCH; q++) {
+      if (valid[q] && lo[q] < A.D) {
+        int64_t j = lo[q];
+        if (A.keys[j] == rep[q]) {
+          VT xj = x[j];
+          ST ar = (ST)(tv[q] * cr[q]), ai = (ST)(tv[q] * ci[q]);
+          orr += ar * (ST)xj.x - ai * (ST)xj.y;
+          oi += ar * (ST)xj.y + ai * (ST)xj.x;
+        }
+      }
+    }
+    if (!more) break;
+  }
+  VT xi = x[i];
+  ST invR = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[i]);
+  ST yr = dg * (ST)xi.x + orr * invR;
+  ST yi = dg * (ST)xi.y + oi * invR;
+  VT out;
+  if (accumulate) {
+    VT y0 = y[i];
+    out.x = y0.x + yr;
+    out.y = y0.y + yi;
+  } else {
+    out.x = yr;
+    out.y = yi;
+  }
+  y[i] = out;
+}
+
+// ---------------------------------------------------------------------------
+// host: eligibility and setup (cached on the operator)
+// ---------------------------------------------------------------------------
+struct FastState {
+  bool checked = false;
+  bool ok = false;
+  FastArgs args;
+  FastTab *d_tab = nullptr;
+  uint32_t *d_keys32 = nullptr;
+  uint32_t *d_dir = nullptr;
+  bool cplx = false;
+};
+
+__global__ void k_keys32(const uint64_t *k64, int64_t D, uint32_t *k32) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D) k32[i] = (uint32_t)k64[i];
+}
+
+__global__ void k_dir32(const uint32_t *keys, int64_t D, int dshift, uint32_t ndir, uint32_t *dir) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > (int64_t)ndir) return;
+  if (b == (int64_t)ndir) {
+    dir[b] = (uint32_t)D;
+    return;
+  }
+  uint64_t key = (uint64_t)b << dshift;
+  int64_t lo = 0, hi = D;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  dir[b] = (uint32_t)lo;
+}
+
+static FastState *fast_state(edk_op *op) {
+  if (!op->fast) op->fast = new FastState();
+  return (FastState *)op->fast;
+}
+
+void edk_fast_release(edk_op *op) {
+  FastState *F = (FastState *)op->fast;
+  if (!F) return;
+  (void)hipFree(F->d_tab);
+  (void)hipFree(F->d_keys32);
+  (void)hipFree(F->d_dir);
+  delete F;
+  op->fast = nullptr;
+}
+
+// Decompose the operator into translation-invariant two-site families.
+static bool find_families(const edk_op *op, FastTab &tab) {
+  const edk_basis *b = op->basis;
+  int L = b->L;
+  if (op->nterms == 0 || op->nterms % L) return false;
+  struct Key {
+    int mat, d;
+  };
+  std::vector<Key> keys;
+  std::vector<std::vector<int>> starts;
+  for (int t = 0; t < op->nterms; t++) {
+    const int *td = &op->h_tdesc[t * EDK_TDESC];
+    if (td[0] != 2) return false;
+    int s1 = L - td[2], s2 = L - td[3];  // sites from field shifts (w == 1)
+    int d = ((s2 - s1) % L + L) % L;
+    if (d == 0) return false;
+    int id = -1;
+    for (size_t k = 0; k < keys.size(); k++)
+      if (keys[k].mat == td[1] && keys[k].d == d) id = (int)k;
+    if (id < 0) {
+      id = (int)keys.size();
+      keys.push_back({td[1], d});
+      starts.push_back({});
+    }
+    starts[id].push_back(s1);
+  }
+  if ((int)keys.size() > FAST_MAXFAM) return false;
+  tab.nfam = (int)keys.size();
+  for (size_t k = 0; k < keys.size(); k++) {
+    std::vector<int> &st = starts[k];
+    if ((int)st.size() != L) return false;
+    std::sort(st.begin(), st.end());
+    for (int i = 0; i < L; i++)
+      if (st[i] != i + 1) return false;
+    FastFam &F = tab.fam[k];
+    F.d = keys[k].d;
+    F.nent = 0;
+    for (int a = 0; a < 4; a++) F.diag[a] = 0.0;
+    int m = keys[k].mat;
+    const int *rp = &op->h_rowptr[op->h_mrow_off[m]];
+    for (int loc = 0; loc < 4; loc++) {
+      for (int q = rp[loc]; q < rp[loc + 1]; q++) {
+        int c = op->h_col[q];
+        double2 v = op->h_val[q];
+        if (v.y != 0.0) return false;
+        if (c == loc) {
+          F.diag[loc] += v.x;
+          continue;
+        }
+        if (F.nent >= FAST_MAXENT) return false;
+        F.ent[F.nent].sel = loc;
+        F.ent[F.nent].flip = loc ^ c;
+        F.ent[F.nent].val = v.x;
+        F.nent++;
+      }
+    }
+  }
+  return true;
+}
+
+static bool fast_setup(edk_op *op) {
+  FastState *S = fast_state(op);
+  if (S->checked) return S->ok;
+  S->checked = true;
+  const edk_basis *b = op->basis;
+  if (b->base != 2 || b->L > 32 || b->uploaded) return false;
+  if (b->kind != EDK_TRANSLATION_PARITY && b->kind != EDK_TRANSLATIONAL && b->kind != EDK_PROJECTED) return false;
+  if (b->D == 0 || b->D >= (1ll << 32) - 1) return false;
+  FastTab tab;
+  if (!find_families(op, tab)) return false;
+  FastArgs &A = S->args;
+  A.kind = b->kind;
+  A.L = b->L;
+  A.A = b->A;
+  A.len = b->len;
+  A.kp = b->kp;
+  A.P = b->P;
+  A.mask = b->L == 32 ? 0xffffffffu : ((1u << b->L) - 1);
+  A.ndiv = 0;
+  for (int p = 1; p <= b->len && A.ndiv < 16; p++)
+    if (b->len % p == 0) A.divs[A.ndiv++] = p;
+  S->cplx = (b->kind == EDK_TRANSLATIONAL && !(b->kp == 0 || 2 * b->kp == b->len));
+  edk_ctx *ctx = op->ctx;
+  if (hipMalloc((void **)&S->d_tab, sizeof(FastTab)) != hipSuccess) return false;
+  if (hipMemcpyAsync(S->d_tab, &tab, sizeof(FastTab), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return false;
+  if (hipMalloc((void **)&S->d_keys32, b->D * sizeof(uint32_t)) != hipSuccess) return false;
+  k_keys32<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(b->d_keys, b->D, S->d_keys32);
+  // directory over the 32-bit keys: about one slot per 16 representatives
+  int64_t target = std::max<int64_t>(1024, b->D / 16);
+  int sh = 0;
+  while (sh < 31 && (int64_t)(b->maxkey >> sh) + 1 > target) sh++;
+  uint32_t ndir = (uint32_t)((b->maxkey >> sh) + 1);
+  if (hipMalloc((void **)&S->d_dir, ((size_t)ndir + 1) * sizeof(uint32_t)) != hipSuccess) return false;
+  k_dir32<<<(unsigned)((ndir + 256) / 256), 256, 0, ctx->stream>>>(S->d_keys32, b->D, sh, ndir, S->d_dir);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
+  A.tab = S->d_tab;
+  A.Rtab = b->d_Rtab;
+  A.ph = b->d_ph;
+  A.keys = S->d_keys32;
+  A.dir = S->d_dir;
+  A.dshift = sh;
+  A.ndir = ndir;
+  A.D = b->D;
+  A.Rrow = b->d_R;
+  S->ok = true;
+  return true;
+}
+
+template <int KIND, bool CPLX>
+static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
+  unsigned grid = (unsigned)((A.D + FAST_TPB - 1) / FAST_TPB);
+  if (dtype == EDK_C64)
+    k_matvec_fast<KIND, CPLX, double2, double><<<grid, FAST_TPB, 0, st>>>(A, (const double2 *)x, (double2 *)y, acc);
+  else
+    k_matvec_fast<KIND, CPLX, float2, float><<<grid, FAST_TPB, 0, st>>>(A, (const float2 *)x, (float2 *)y, acc);
+}
+
+bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
+  if (nb != 1 || op->disable_fast) return false;
+  if (!fast_setup(op)) return false;
+  FastState *S = (FastState *)op->fast;
+  const FastArgs &A = S->args;
+  hipStream_t st = op->ctx->stream;
+  switch (A.kind) {
+    case EDK_TRANSLATION_PARITY: launch_fast<EDK_TRANSLATION_PARITY, false>(A, dtype, y, x, accumulate, st); break;
+    case EDK_TRANSLATIONAL:
+      if (S->cplx) launch_fast<EDK_TRANSLATIONAL, true>(A, dtype, y, x, accumulate, st);
+      else launch_fast<EDK_TRANSLATIONAL, false>(A, dtype, y, x, accumulate, st);
+      break;
+    case EDK_PROJECTED: launch_fast<EDK_PROJECTED, false>(A, dtype, y, x, accumulate, st); break;
+    default: return false;
+  }
+  return hipGetLastError() == hipSuccess;
+}

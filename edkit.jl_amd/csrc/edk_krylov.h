@@ -1,0 +1,30 @@
+// edk_krylov.h -- device BLAS-1 / reorthogonalisation helpers shared by the
+// Lanczos and Krylov-evolution drivers.
+#pragma once
+#include "edk_matvec.h"
+
+#define EDK_KRYLOV_MAXM 1024
+
+// Deterministic FP64 vector kernels over length-n vectors of one dtype.
+struct KrylovWork {
+  edk_ctx *ctx = nullptr;
+  int64_t n = 0;
+  int dtype = EDK_C64;
+  int nblk = 1;
+  double2 *d_part = nullptr;  // block partials
+  double2 *d_s = nullptr;     // reduced coefficients
+  double2 *h_s = nullptr;     // pinned host mirror
+  int init(edk_ctx *c, int64_t n, int dtype, int maxm);
+  void release();
+  size_t esize() const;
+  int dot(const void *a, const void *b, double2 *out);  // sum conj(a) b (b == null: |a|^2)
+  int norm(const void *a, double *out);
+  int axpy(void *y, const void *x, double2 alpha);
+  int scale(void *y, const void *x, double s);
+  int reorth(const void *V, int64_t ldv, int j, void *w);
+  int combine(const void *V, int64_t ldv, int j, const double2 *c, double scale, void *out, int out_dtype);
+};
+
+int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z);
+int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta, int *m,
+                   int m_target, int64_t *matvecs, int ncols);

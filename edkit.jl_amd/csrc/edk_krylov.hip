@@ -1,0 +1,754 @@
+// edk_krylov.hip -- device-resident Lanczos ground state and Krylov
+// exp(-iHt)psi (KrylovEvolutionCache, TimeEvolution.jl:84-649).
+//
+// Vectors (the Krylov basis V, the work vector w, the anchor state) stay in
+// HBM in the caller's precision (ComplexF64, or ComplexF32 for the precision
+// form); every inner product, norm, reorthogonalisation coefficient and all
+// tridiagonal/reduced-space arithmetic is FP64.  Reductions are two-stage
+// with a grid that depends only on the vector length, and the second stage
+// sums the block partials in a fixed order, so a result is bitwise the same
+// whatever else runs on the device.
+//
+// Reorthogonalisation is classical Gram-Schmidt applied twice (CGS2) as a
+// pair of GEMV sweeps over V; the reference runs the same two sweeps in
+// modified-Gram-Schmidt order (TimeEvolution.jl:275-283).  Both give a basis
+// orthogonal to working precision; values differ at rounding level.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "edk_krylov.h"
+
+#define RED_TPB 256
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <typename VT>
+__device__ __forceinline__ double2 ld(const VT *p, int64_t i) {
+  VT v = p[i];
+  return make_double2((double)v.x, (double)v.y);
+}
+template <typename VT>
+__device__ __forceinline__ void st(VT *p, int64_t i, double2 v) {
+  VT o;
+  o.x = (decltype(o.x))v.x;
+  o.y = (decltype(o.y))v.y;
+  p[i] = o;
+}
+
+__device__ __forceinline__ double2 block_sum2(double2 v) {
+  __shared__ double2 sm[RED_TPB / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_down(v.x, o, 64);
+    v.y += __shfl_down(v.y, o, 64);
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sm[wid] = v;
+  __syncthreads();
+  double2 r = make_double2(0, 0);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < RED_TPB / 64; q++) {
+      r.x += sm[q].x;
+      r.y += sm[q].y;
+    }
+  return r;
+}
+
+// partial[b] = sum over block b's rows of conj(a) * b  (or |a|^2 when b == null)
+template <typename VT>
+__global__ void __launch_bounds__(RED_TPB) k_dot_part(const VT *a, const VT *b, int64_t n, double2 *part) {
+  double2 acc = make_double2(0, 0);
+  for (int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_TPB) {
+    double2 x = ld(a, i);
+    if (b) {
+      double2 y = ld(b, i);
+      acc.x += x.x * y.x + x.y * y.y;
+      acc.y += x.x * y.y - x.y * y.x;
+    } else {
+      acc.x += x.x * x.x + x.y * x.y;
+    }
+  }
+  double2 r = block_sum2(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// out[c] = sum_b part[c * nb + b], fixed order; one block per c
+__global__ void __launch_bounds__(RED_TPB) k_reduce_part(const double2 *part, int nb, double2 *out) {
+  double2 acc = make_double2(0, 0);
+  const double2 *p = part + (int64_t)blockIdx.x * nb;
+  for (int b = threadIdx.x; b < nb; b += RED_TPB) {
+    acc.x += p[b].x;
+    acc.y += p[b].y;
+  }
+  double2 r = block_sum2(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
+// y += alpha * x
+template <typename VT>
+__global__ void k_axpy(VT *y, const VT *x, double2 alpha, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 a = ld(y, i), b = ld(x, i);
+  a.x += alpha.x * b.x - alpha.y * b.y;
+  a.y += alpha.x * b.y + alpha.y * b.x;
+  st(y, i, a);
+}
+
+// y = x * s  (real s)
+template <typename VT, typename VS>
+__global__ void k_scale(VT *y, const VS *x, double s, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 b = ld(x, i);
+  st(y, i, make_double2(b.x * s, b.y * s));
+}
+
+// part[c * nblk + b] = block b's share of conj(V[:, c]) . w,  c < j
+template <typename VT>
+__global__ void __launch_bounds__(RED_TPB) k_gemv_t(const VT *V, int64_t ld_v, int j, const VT *w, int64_t n,
+                                                    double2 *part) {
+  for (int c = 0; c < j; c++) {
+    const VT *vc = V + (int64_t)c * ld_v;
+    double2 acc = make_double2(0, 0);
+    for (int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_TPB) {
+      double2 x = ld(vc, i), y = ld(w, i);
+      acc.x += x.x * y.x + x.y * y.y;
+      acc.y += x.x * y.y - x.y * y.x;
+    }
+    double2 r = block_sum2(acc);
+    if (threadIdx.x == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+// out = scale_in * out_in - sum_c V[:, c] s[c]   (mode 0: w -= V s)
+// out = scale * sum_c V[:, c] s[c]               (mode 1: reconstruct)
+template <typename VT, typename VO>
+__global__ void __launch_bounds__(RED_TPB) k_gemv_n(const VT *V, int64_t ld_v, int j, const double2 *s, VO *out,
+                                                    int64_t n, int mode, double scale) {
+  __shared__ double2 sc[EDK_KRYLOV_MAXM + 1];
+  for (int c = threadIdx.x; c < j; c += RED_TPB) sc[c] = s[c];
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x;
+  if (i >= n) return;
+  double2 acc = make_double2(0, 0);
+  for (int c = 0; c < j; c++) {
+    double2 v = ld(V + (int64_t)c * ld_v, i);
+    acc.x += v.x * sc[c].x - v.y * sc[c].y;
+    acc.y += v.x * sc[c].y + v.y * sc[c].x;
+  }
+  if (mode == 0) {
+    double2 o = ld(out, i);
+    st(out, i, make_double2(o.x - acc.x, o.y - acc.y));
+  } else {
+    st(out, i, make_double2(acc.x * scale, acc.y * scale));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+static int red_blocks(int64_t n) {
+  int64_t b = (n + RED_TPB * 4 - 1) / (RED_TPB * 4);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
+}
+
+int KrylovWork::init(edk_ctx *c, int64_t n_, int dtype_, int maxm) {
+  ctx = c;
+  n = n_;
+  dtype = dtype_;
+  nblk = red_blocks(n);
+  size_t pb = (size_t)nblk * (maxm + 2) * sizeof(double2);
+  if (hipMalloc((void **)&d_part, pb) != hipSuccess) return edk_fail(EDK_ERR_OOM, "krylov partials");
+  if (hipMalloc((void **)&d_s, (maxm + 2) * sizeof(double2)) != hipSuccess)
+    return edk_fail(EDK_ERR_OOM, "krylov coefficients");
+  if (hipHostMalloc((void **)&h_s, (maxm + 2) * sizeof(double2)) != hipSuccess)
+    return edk_fail(EDK_ERR_OOM, "krylov host buffer");
+  return EDK_OK;
+}
+
+void KrylovWork::release() {
+  hipFree(d_part);
+  hipFree(d_s);
+  if (h_s) hipHostFree(h_s);
+  d_part = nullptr;
+  d_s = nullptr;
+  h_s = nullptr;
+}
+
+size_t KrylovWork::esize() const { return dtype == EDK_C64 ? 16 : 8; }
+
+#define DISPATCH(VTNAME, ...)                      \
+  do {                                             \
+    if (dtype == EDK_C64) {                        \
+      using VTNAME = double2;                      \
+      __VA_ARGS__;                                 \
+    } else {                                       \
+      using VTNAME = float2;                       \
+      __VA_ARGS__;                                 \
+    }                                              \
+  } while (0)
+
+int KrylovWork::dot(const void *a, const void *b, double2 *out) {
+  hipStream_t s = ctx->stream;
+  DISPATCH(VT, k_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)a, (const VT *)b, n, d_part));
+  k_reduce_part<<<1, RED_TPB, 0, s>>>(d_part, nblk, d_s);
+  EDK_HIP(hipGetLastError());
+  EDK_HIP(hipMemcpyAsync(h_s, d_s, sizeof(double2), hipMemcpyDeviceToHost, s));
+  EDK_HIP(hipStreamSynchronize(s));
+  *out = h_s[0];
+  return EDK_OK;
+}
+
+int KrylovWork::norm(const void *a, double *out) {
+  double2 r;
+  if (int rc = dot(a, nullptr, &r)) return rc;
+  *out = std::sqrt(r.x);
+  return EDK_OK;
+}
+
+int KrylovWork::axpy(void *y, const void *x, double2 alpha) {
+  unsigned g = (unsigned)((n + 255) / 256);
+  DISPATCH(VT, k_axpy<VT><<<g, 256, 0, ctx->stream>>>((VT *)y, (const VT *)x, alpha, n));
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+
+int KrylovWork::scale(void *y, const void *x, double s) {
+  unsigned g = (unsigned)((n + 255) / 256);
+  DISPATCH(VT, k_scale<VT, VT><<<g, 256, 0, ctx->stream>>>((VT *)y, (const VT *)x, s, n));
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+
+// w -= V[:, :j] (V[:, :j]^H w), twice
+int KrylovWork::reorth(const void *V, int64_t ldv, int j, void *w) {
+  hipStream_t s = ctx->stream;
+  for (int sweep = 0; sweep < 2; sweep++) {
+    DISPATCH(VT, k_gemv_t<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)V, ldv, j, (const VT *)w, n, d_part));
+    k_reduce_part<<<j, RED_TPB, 0, s>>>(d_part, nblk, d_s);
+    unsigned g = (unsigned)((n + RED_TPB - 1) / RED_TPB);
+    DISPATCH(VT, k_gemv_n<VT, VT><<<g, RED_TPB, 0, s>>>((const VT *)V, ldv, j, d_s, (VT *)w, n, 0, 0.0));
+  }
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+
+// out = scale * V[:, :j] c   (c: host complex coefficients)
+int KrylovWork::combine(const void *V, int64_t ldv, int j, const double2 *c, double scale, void *out, int out_dtype) {
+  hipStream_t s = ctx->stream;
+  memcpy(h_s, c, j * sizeof(double2));
+  EDK_HIP(hipMemcpyAsync(d_s, h_s, j * sizeof(double2), hipMemcpyHostToDevice, s));
+  unsigned g = (unsigned)((n + RED_TPB - 1) / RED_TPB);
+  if (out_dtype == EDK_C64) {
+    DISPATCH(VT, k_gemv_n<VT, double2><<<g, RED_TPB, 0, s>>>((const VT *)V, ldv, j, d_s, (double2 *)out, n, 1, scale));
+  } else {
+    DISPATCH(VT, k_gemv_n<VT, float2><<<g, RED_TPB, 0, s>>>((const VT *)V, ldv, j, d_s, (float2 *)out, n, 1, scale));
+  }
+  EDK_HIP(hipGetLastError());
+  // the host buffer is reused by the next call: wait for the copy
+  EDK_HIP(hipStreamSynchronize(s));
+  return EDK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// symmetric tridiagonal eigensolver (implicit QL with Wilkinson shift, the
+// textbook method of Golub & Van Loan 8.3), FP64
+// ---------------------------------------------------------------------------
+int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z) {
+  std::vector<double> e(n + 1, 0.0);
+  for (int i = 0; i < n; i++) d[i] = alpha[i];
+  for (int i = 0; i + 1 < n; i++) e[i] = beta[i];
+  for (int i = 0; i < n * n; i++) Z[i] = 0.0;
+  for (int i = 0; i < n; i++) Z[i + (size_t)i * n] = 1.0;
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < n; l++) {
+    int iter = 0, m;
+    do {
+      for (m = l; m < n - 1; m++) {
+        double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= eps * dd) break;
+      }
+      if (m != l) {
+        if (iter++ == 100) return edk_fail(EDK_ERR_NUMERIC, "tridiagonal eigensolver did not converge");
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; i--) {
+          double f = s * e[i], bb = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * bb;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - bb;
+          for (int k = 0; k < n; k++) {
+            double zf = Z[k + (size_t)(i + 1) * n];
+            Z[k + (size_t)(i + 1) * n] = s * Z[k + (size_t)i * n] + c * zf;
+            Z[k + (size_t)i * n] = c * Z[k + (size_t)i * n] - s * zf;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  for (int i = 0; i < n - 1; i++) {
+    int k = i;
+    for (int j = i + 1; j < n; j++)
+      if (d[j] < d[k]) k = j;
+    if (k != i) {
+      std::swap(d[i], d[k]);
+      for (int q = 0; q < n; q++) std::swap(Z[q + (size_t)i * n], Z[q + (size_t)k * n]);
+    }
+  }
+  return EDK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos step: the _lanczos_build! recurrence (TimeEvolution.jl:250-306)
+// ---------------------------------------------------------------------------
+// Extends the basis V (columns 0..) from m_start to at most m_target steps.
+// Returns 1 on lucky breakdown (beta set to 0), 0 otherwise; <0 on error.
+int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta,
+                   int *m, int m_target, int64_t *matvecs, int ncols) {
+  size_t es = K.esize();
+  for (int j = *m + 1; j <= m_target; j++) {
+    char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
+    if (int rc = edk_matvec_launch(op, K.dtype, w, vj, 1, 0)) return -rc;
+    (*matvecs)++;
+    if (j > 1) {
+      char *vjm1 = (char *)V + (size_t)(j - 2) * ldv * es;
+      if (int rc = K.axpy(w, vjm1, make_double2(-beta[j - 2], 0.0))) return -rc;
+    }
+    double2 d;
+    if (int rc = K.dot(vj, w, &d)) return -rc;
+    double aj = d.x;
+    alpha[j - 1] = aj;
+    if (int rc = K.axpy(w, vj, make_double2(-aj, 0.0))) return -rc;
+    if (int rc = K.reorth(V, ldv, j, w)) return -rc;
+    double bj;
+    if (int rc = K.norm(w, &bj)) return -rc;
+    beta[j - 1] = bj;
+    const double sq = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+    if (bj <= sq * std::max(1.0, std::fabs(aj))) {
+      *m = j;
+      beta[j - 1] = 0.0;
+      return 1;
+    }
+    *m = j;
+    if (j + 1 <= ncols) {
+      char *vn = (char *)V + (size_t)j * ldv * es;
+      if (int rc = K.scale(vn, w, 1.0 / bj)) return -rc;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos ground state
+// ---------------------------------------------------------------------------
+extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, double tol, int refine, double *E0,
+                                  double *resid, int *steps, void *vec_out) {
+  if (!op || !x0 || !E0 || !resid || m < 1) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground: bad argument");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "bad dtype");
+  if (m > EDK_KRYLOV_MAXM) return edk_fail(EDK_ERR_UNSUPPORTED, "Lanczos dimension above 1024");
+  edk_ctx *ctx = op->ctx;
+  int64_t D = op->basis->D;
+  if (D < 1) return edk_fail(EDK_ERR_ARG, "empty sector");
+  if (m > D) m = (int)D;
+  EDK_HIP(hipSetDevice(ctx->device));
+  KrylovWork K;
+  if (int rc = K.init(ctx, D, dtype, m)) {
+    K.release();
+    return rc;
+  }
+  size_t es = K.esize();
+  void *V = nullptr, *w = nullptr;
+  if (hipMalloc(&V, (size_t)D * (m + 1) * es) != hipSuccess || hipMalloc(&w, (size_t)D * es) != hipSuccess) {
+    hipFree(V);
+    K.release();
+    return edk_fail(EDK_ERR_OOM, "Lanczos basis does not fit in device memory");
+  }
+  int rc = EDK_OK;
+  double nrm = 0;
+  std::vector<double> alpha(m, 0.0), beta(m, 0.0), lam(m), Q((size_t)m * m);
+  int mm = 0;
+  int64_t mv = 0;
+  rc = K.norm(x0, &nrm);
+  if (!rc && !(nrm > 0)) rc = edk_fail(EDK_ERR_NUMERIC, "Initial state has zero norm");
+  if (!rc) rc = K.scale(V, x0, 1.0 / nrm);
+  double e0 = 0, res = 0;
+  for (int j = 1; !rc && j <= m; j++) {
+    int br = lanczos_extend(op, K, V, D, w, alpha.data(), beta.data(), &mm, j, &mv, m + 1);
+    if (br < 0) {
+      rc = -br;
+      break;
+    }
+    if ((rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data()))) break;
+    e0 = lam[0];
+    res = br ? 0.0 : std::fabs(beta[mm - 1] * Q[(mm - 1)]);
+    if (br || (tol > 0 && res <= tol)) break;
+  }
+  if (!rc && (refine || vec_out)) {
+    std::vector<double2> c(mm);
+    for (int k = 0; k < mm; k++) c[k] = make_double2(Q[k], 0.0);
+    // Ritz vector in FP64 storage (accumulated in FP64 from the basis)
+    void *y = nullptr, *hy = nullptr;
+    if (hipMalloc(&y, (size_t)D * 16) != hipSuccess) rc = edk_fail(EDK_ERR_OOM, "Ritz vector");
+    if (!rc) rc = K.combine(V, D, mm, c.data(), 1.0, y, EDK_C64);
+    if (!rc && refine) {
+      // FP64 Rayleigh quotient and residual with the ComplexF64 matvec
+      KrylovWork K64;
+      if (hipMalloc(&hy, (size_t)D * 16) != hipSuccess) rc = edk_fail(EDK_ERR_OOM, "refinement vector");
+      if (!rc) rc = K64.init(ctx, D, EDK_C64, 1);
+      if (!rc) rc = edk_matvec_launch(op, EDK_C64, hy, y, 1, 0);
+      double2 num, den;
+      if (!rc) rc = K64.dot(y, hy, &num);
+      if (!rc) rc = K64.dot(y, y, &den);
+      if (!rc) {
+        double E = num.x / den.x;
+        rc = K64.axpy(hy, y, make_double2(-E, 0.0));
+        double rn = 0;
+        if (!rc) rc = K64.norm(hy, &rn);
+        if (!rc) {
+          e0 = E;
+          res = rn / std::sqrt(den.x);
+        }
+      }
+      K64.release();
+      hipFree(hy);
+    }
+    if (!rc && vec_out) {
+      if (dtype == EDK_C64) rc = hipMemcpyAsync(vec_out, y, (size_t)D * 16, hipMemcpyDeviceToDevice, ctx->stream) == hipSuccess ? EDK_OK : edk_fail(EDK_ERR_HIP, "copy Ritz vector");
+      else {
+        k_scale<float2, double2><<<(unsigned)((D + 255) / 256), 256, 0, ctx->stream>>>((float2 *)vec_out, (const double2 *)y, 1.0, D);
+      }
+      if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = edk_fail(EDK_ERR_HIP, "Ritz vector sync");
+    }
+    hipFree(y);
+  }
+  hipStreamSynchronize(ctx->stream);
+  hipFree(V);
+  hipFree(w);
+  K.release();
+  if (rc) return rc;
+  *E0 = e0;
+  *resid = res;
+  if (steps) *steps = mm;
+  return EDK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// KrylovEvolutionCache
+// ---------------------------------------------------------------------------
+struct edk_krylov {
+  edk_op *H;
+  int dtype;
+  int64_t N;
+  double t_anchor = 0, t_latest = 0;
+  void *psi_anchor = nullptr;
+  double norm_anchor = 0;
+  void *V = nullptr;  // N x (m_max+1)
+  void *w = nullptr;
+  std::vector<double> alpha, beta, lambda, Q, Qe1, Qem;
+  int m = 0;
+  double tol;
+  int m_init, m_max, extend_step, nsample, bisect_iters, extend_basis, normalize_output;
+  edk_krylov_diag diag;
+  std::vector<double> accepted;
+  KrylovWork K;
+};
+
+extern "C" void edk_krylov_default_params(edk_krylov_params *p) {
+  p->tol = 1e-10;
+  p->m_init = 30;
+  p->m_max = 60;
+  p->extend_step = 10;
+  p->nsample = 13;
+  p->bisect_iters = 12;
+  p->extend_basis = 1;
+  p->normalize_output = 0;
+}
+
+// _build_reduced! (TimeEvolution.jl:311-323)
+static int build_reduced(edk_krylov *c) {
+  int m = c->m;
+  if (m < 1) return edk_fail(EDK_ERR_NUMERIC, "Cannot build reduced propagator for empty basis");
+  c->lambda.assign(m, 0);
+  c->Q.assign((size_t)m * m, 0);
+  c->Qe1.assign(m, 0);
+  c->Qem.assign(m, 0);
+  if (int rc = edk_tridiag_eig(m, c->alpha.data(), c->beta.data(), c->lambda.data(), c->Q.data())) return rc;
+  for (int k = 0; k < m; k++) {
+    c->Qe1[k] = c->Q[0 + (size_t)k * m];
+    c->Qem[k] = c->Q[(m - 1) + (size_t)k * m];
+  }
+  return EDK_OK;
+}
+
+static inline double2 cisd(double x) { return make_double2(std::cos(x), std::sin(x)); }
+
+// _defect_monitor (TimeEvolution.jl:341-350)
+static double defect(const edk_krylov *c, double tau) {
+  int m = c->m;
+  double bm = c->beta[m - 1];
+  if (bm == 0.0) return 0.0;
+  double ar = 0, ai = 0;
+  for (int k = 0; k < m; k++) {
+    double2 z = cisd(-tau * c->lambda[k]);
+    ar += c->Qem[k] * z.x * c->Qe1[k];
+    ai += c->Qem[k] * z.y * c->Qe1[k];
+  }
+  return std::fabs(bm) * std::hypot(ar, ai);
+}
+
+// _reconstruct_state! (TimeEvolution.jl:352-362) with _reduced_coeffs (:329-338)
+static int reconstruct(edk_krylov *c, double tau, void *out) {
+  int m = c->m;
+  std::vector<double2> ph(m), cc(m);
+  for (int k = 0; k < m; k++) {
+    double2 z = cisd(-tau * c->lambda[k]);
+    ph[k] = make_double2(z.x * c->Qe1[k], z.y * c->Qe1[k]);
+  }
+  for (int i = 0; i < m; i++) {
+    double sr = 0, si = 0;
+    for (int k = 0; k < m; k++) {
+      sr += c->Q[i + (size_t)k * m] * ph[k].x;
+      si += c->Q[i + (size_t)k * m] * ph[k].y;
+    }
+    cc[i] = make_double2(sr, si);
+  }
+  if (int rc = c->K.combine(c->V, c->N, m, cc.data(), c->norm_anchor, out, c->dtype)) return rc;
+  if (c->normalize_output) {
+    double nrm;
+    if (int rc = c->K.norm(out, &nrm)) return rc;
+    if (nrm > 0)
+      if (int rc = c->K.scale(out, out, 1.0 / nrm)) return rc;
+  }
+  return EDK_OK;
+}
+
+// _effective_nsample (TimeEvolution.jl:383-394)
+static int eff_nsample(const edk_krylov *c, double tau) {
+  int base = c->nsample;
+  if (c->m <= 1) return base;
+  double om = c->lambda[c->m - 1] - c->lambda[0];
+  if (om <= 0) return base;
+  int nn = (int)std::ceil(4.0 * om * tau / M_PI) + 1;
+  return std::min(std::max(nn, base), 513);
+}
+
+// _max_valid_interval (TimeEvolution.jl:396-429)
+static double max_valid(const edk_krylov *c, double tau) {
+  if (!(tau > 0)) return 0.0;
+  if (std::fabs(c->beta[c->m - 1]) <= c->tol) return tau;
+  int n = eff_nsample(c, tau);
+  double prev = 0.0;
+  for (int k = 1; k <= n; k++) {
+    double x = tau * (double)(k - 1) / (double)(n - 1);
+    double eta = defect(c, x);
+    if (!std::isfinite(eta) || eta > c->tol) {
+      if (k == 1) return 0.0;
+      double lo = prev, hi = x;
+      for (int q = 0; q < c->bisect_iters; q++) {
+        double mid = 0.5 * (lo + hi);
+        if (defect(c, mid) > c->tol) hi = mid;
+        else lo = mid;
+      }
+      return lo;
+    }
+    prev = x;
+  }
+  return tau;
+}
+
+static int lanczos_build(edk_krylov *c, int m_target) {
+  m_target = std::min(m_target, c->m_max);
+  int r = lanczos_extend(c->H, c->K, c->V, c->N, c->w, c->alpha.data(), c->beta.data(), &c->m, m_target,
+                         &c->diag.matvecs, c->m_max + 1);
+  return r < 0 ? -r : EDK_OK;
+}
+
+// _extend_basis! (TimeEvolution.jl:434-447); *did = 1 when the basis grew
+static int extend(edk_krylov *c, int *did) {
+  *did = 0;
+  if (!c->extend_basis || c->m >= c->m_max) return EDK_OK;
+  int old = c->m;
+  if (int rc = lanczos_build(c, std::min(c->m + c->extend_step, c->m_max))) return rc;
+  if (c->m > old) {
+    if (int rc = build_reduced(c)) return rc;
+    c->diag.basis_extensions++;
+    c->diag.max_dim_used = std::max<int64_t>(c->diag.max_dim_used, c->m);
+    *did = 1;
+  }
+  return EDK_OK;
+}
+
+// _restart_from_new_anchor! (TimeEvolution.jl:449-477)
+static int restart(edk_krylov *c, double tau) {
+  if (int rc = reconstruct(c, tau, c->psi_anchor)) return rc;
+  c->t_anchor += tau;
+  double nrm;
+  if (int rc = c->K.norm(c->psi_anchor, &nrm)) return rc;
+  if (!(nrm > 0)) return edk_fail(EDK_ERR_NUMERIC, "Restart failed: anchor state has zero norm");
+  c->norm_anchor = nrm;
+  std::fill(c->alpha.begin(), c->alpha.end(), 0.0);
+  std::fill(c->beta.begin(), c->beta.end(), 0.0);
+  c->m = 0;
+  if (int rc = c->K.scale(c->V, c->psi_anchor, 1.0 / nrm)) return rc;
+  if (int rc = lanczos_build(c, c->m_init)) return rc;
+  if (int rc = build_reduced(c)) return rc;
+  c->diag.basis_builds++;
+  c->diag.restarts++;
+  c->diag.max_dim_used = std::max<int64_t>(c->diag.max_dim_used, c->m);
+  return EDK_OK;
+}
+
+// _ensure_valid_up_to! (TimeEvolution.jl:486-520)
+static int ensure_valid(edk_krylov *c, double target) {
+  double rem = target;
+  if (rem <= 0) return EDK_OK;
+  while (true) {
+    double v = max_valid(c, rem);
+    if (v >= rem) return EDK_OK;
+    int did = 0;
+    if (c->extend_basis && c->m < c->m_max) {
+      if (int rc = extend(c, &did)) return rc;
+      if (did) continue;
+    }
+    if (v <= 0) {
+      if (c->m < c->m_max) {
+        if (int rc = extend(c, &did)) return rc;
+        if (did) continue;
+      }
+      return edk_fail(EDK_ERR_NUMERIC,
+                      "KrylovEvolutionCache: basis is not valid at tau=0; consider increasing m_max or loosening tol");
+    }
+    c->accepted.push_back(v);
+    c->diag.n_accepted_intervals++;
+    if (int rc = restart(c, v)) return rc;
+    rem -= v;
+  }
+}
+
+static void krylov_free(edk_krylov *c) {
+  if (!c) return;
+  hipFree(c->psi_anchor);
+  hipFree(c->V);
+  hipFree(c->w);
+  c->K.release();
+  delete c;
+}
+
+extern "C" int edk_krylov_create(edk_op *op, int dtype, const void *psi0, const edk_krylov_params *p,
+                                 edk_krylov **out) {
+  if (!op || !psi0 || !out) return edk_fail(EDK_ERR_ARG, "edk_krylov_create: null argument");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "bad dtype");
+  edk_krylov_params prm;
+  edk_krylov_default_params(&prm);
+  if (p) prm = *p;
+  int64_t N = op->basis->D;
+  // argument checks of KrylovEvolutionCache (TimeEvolution.jl:178-185)
+  if (N <= 0) return edk_fail(EDK_ERR_ARG, "Initial state must be non-empty");
+  if (prm.m_init < 1) return edk_fail(EDK_ERR_ARG, "m_init must be >= 1");
+  if (prm.m_max < prm.m_init) return edk_fail(EDK_ERR_ARG, "m_max must be >= m_init");
+  if (prm.m_max > N) prm.m_max = (int)N;
+  if (prm.m_init > prm.m_max) prm.m_init = prm.m_max;
+  if (prm.nsample < 2) return edk_fail(EDK_ERR_ARG, "nsample must be >= 2");
+  if (prm.bisect_iters < 0) return edk_fail(EDK_ERR_ARG, "bisect_iters must be >= 0");
+  if (prm.extend_step < 1) return edk_fail(EDK_ERR_ARG, "extend_step must be >= 1");
+  if (prm.m_max > EDK_KRYLOV_MAXM) return edk_fail(EDK_ERR_UNSUPPORTED, "m_max above 1024");
+  edk_ctx *ctx = op->ctx;
+  EDK_HIP(hipSetDevice(ctx->device));
+  edk_krylov *c = new edk_krylov();
+  c->H = op;
+  c->dtype = dtype;
+  c->N = N;
+  c->tol = prm.tol;
+  c->m_init = prm.m_init;
+  c->m_max = prm.m_max;
+  c->extend_step = prm.extend_step;
+  c->nsample = prm.nsample;
+  c->bisect_iters = prm.bisect_iters;
+  c->extend_basis = prm.extend_basis;
+  c->normalize_output = prm.normalize_output;
+  memset(&c->diag, 0, sizeof(c->diag));
+  c->alpha.assign(c->m_max, 0.0);
+  c->beta.assign(c->m_max, 0.0);
+  int rc = c->K.init(ctx, N, dtype, c->m_max + 1);
+  size_t es = c->K.esize();
+  if (!rc && (hipMalloc(&c->psi_anchor, N * es) != hipSuccess || hipMalloc(&c->V, (size_t)N * (c->m_max + 1) * es) != hipSuccess ||
+              hipMalloc(&c->w, N * es) != hipSuccess))
+    rc = edk_fail(EDK_ERR_OOM, "Krylov cache does not fit in device memory");
+  if (!rc && hipMemcpyAsync(c->psi_anchor, psi0, N * es, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+    rc = edk_fail(EDK_ERR_HIP, "copy psi0");
+  double nrm = 0;
+  if (!rc) rc = c->K.norm(c->psi_anchor, &nrm);
+  if (!rc && !(nrm > 0)) rc = edk_fail(EDK_ERR_NUMERIC, "Initial state has zero norm");
+  if (!rc) {
+    c->norm_anchor = nrm;
+    rc = c->K.scale(c->V, c->psi_anchor, 1.0 / nrm);
+  }
+  if (!rc) rc = lanczos_build(c, c->m_init);
+  if (!rc) rc = build_reduced(c);
+  if (rc) {
+    krylov_free(c);
+    return rc;
+  }
+  c->diag.basis_builds = 1;
+  c->diag.max_dim_used = c->m;
+  *out = c;
+  return EDK_OK;
+}
+
+extern "C" int edk_krylov_evolve(edk_krylov *c, double t, void *out) {
+  if (!c || !out) return edk_fail(EDK_ERR_ARG, "edk_krylov_evolve: null argument");
+  EDK_HIP(hipSetDevice(c->H->ctx->device));
+  if (t < c->t_latest)
+    return edk_fail(EDK_ERR_ARG, "timeevolve!: requested time precedes the last served time; backward evolution is not supported");
+  double tau = t - c->t_anchor;
+  if (tau < 0) return edk_fail(EDK_ERR_ARG, "timeevolve!: requested time precedes anchor time");
+  if (int rc = ensure_valid(c, tau)) return rc;
+  if (int rc = reconstruct(c, t - c->t_anchor, out)) return rc;
+  c->t_latest = t;
+  c->diag.total_times_served++;
+  EDK_HIP(hipStreamSynchronize(c->H->ctx->stream));
+  return EDK_OK;
+}
+
+extern "C" int edk_krylov_diagnostics(edk_krylov *c, edk_krylov_diag *d) {
+  if (!c || !d) return edk_fail(EDK_ERR_ARG, "null argument");
+  *d = c->diag;
+  return EDK_OK;
+}
+
+extern "C" int edk_krylov_accepted_intervals(edk_krylov *c, double *out, int64_t cap, int64_t *n) {
+  if (!c || !n) return edk_fail(EDK_ERR_ARG, "null argument");
+  *n = (int64_t)c->accepted.size();
+  for (int64_t i = 0; i < std::min<int64_t>(cap, *n); i++) out[i] = c->accepted[i];
+  return EDK_OK;
+}
+
+extern "C" int edk_krylov_destroy(edk_krylov *c) {
+  if (!c) return EDK_OK;
+  hipSetDevice(c->H->ctx->device);
+  hipStreamSynchronize(c->H->ctx->stream);
+  krylov_free(c);
+  return EDK_OK;
+}

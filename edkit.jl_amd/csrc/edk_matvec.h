@@ -1,0 +1,60 @@
+// edk_matvec.h -- operator tables and matvec kernel arguments.
+#pragma once
+#include "edk_internal.h"
+
+#define EDK_TDESC (2 + EDK_MAXSITES)  // per term: nsites, matrix id, field shifts
+
+struct TermsDev {
+  int nterms;
+  const int *tdesc;        // EDK_TDESC ints per term
+  const uint64_t *tpw;     // base^(L-site) per term site (content weights)
+  const int *mrow_off;     // per distinct matrix: offset of its row pointers
+  const int *rowptr;       // CSR row pointers into col/val
+  const int *col;          // local column c of M[r, c]
+  const double2 *val;      // M[r, c]
+};
+
+struct MatvecArgs {
+  CanonDev C;
+  SearchDev S;
+  TermsDev T;
+  const double *R;   // basis norms b.R
+  int kind;
+  int64_t D;
+  int L, base, w;
+  int orbit_norms;   // 1: norms follow from the orbit (library-built table)
+};
+
+struct edk_op {
+  edk_ctx *ctx;
+  edk_basis *basis;
+  bool all_real;
+  int nterms;
+  TermsDev terms;
+  int *d_tdesc = nullptr;
+  uint64_t *d_tpw = nullptr;
+  int *d_mrow_off = nullptr, *d_rowptr = nullptr, *d_col = nullptr;
+  double2 *d_val = nullptr;
+  void *d_scratch = nullptr;
+  void *fast = nullptr;       // state of the specialised kernel (edk_fast.hip)
+  bool disable_fast = false;  // force the generic kernel (tests compare both)
+  size_t scratch_bytes = 0;
+  std::vector<int> h_tdesc, h_mrow_off, h_rowptr, h_col;
+  std::vector<double2> h_val;
+  MatvecArgs args() const;
+};
+
+__device__ __forceinline__ uint64_t edk_content_to_packed_d(uint64_t c, int L, int base, int w) {
+  if (base == 2) return c;
+  uint64_t s = 0;
+  for (int q = 0; q < L; q++) {
+    s |= (c % (uint64_t)base) << (q * w);
+    c /= (uint64_t)base;
+  }
+  return s;
+}
+
+// Specialised kernels; returns true when it handled the call.
+void edk_fast_release(edk_op *op);
+bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate);
+int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate);

@@ -1,0 +1,362 @@
+// edk_matvec.hip -- Operator upload and the atomics-free gather matvec.
+//
+// The reference applies H column by column and scatters (colmn!,
+// Operator.jl:426-465): for column j, every nonzero (row, val) of every local
+// term at loc_t(a_j) rewrites the term's digits and adds
+//     y[pos] += (x_j / R_j) * (phase * R_pos) * val.
+// The device computes each output row instead (gather): for row i and term t
+// it walks row loc_t(a_i) of M_t, i.e. the entries M_t[loc, c], builds
+// b' = a_i with the term's sites set to c, canonicalises b' to (rep, phase),
+// finds rep's position j in the sorted table and adds
+//     y_i += M_t[loc, c] * conj(phase) * (R_j / R_i) * x_j ,
+// which is the (i, j) element of the same sector matrix when the operator
+// commutes with the symmetry group.  Diagonal local entries (c == loc) give
+// b' = a_i and contribute M_t[loc, loc] * x_i without any lookup.  Each row is
+// written once by one lane: no atomics, and the summation order per row is
+// fixed, so results are bitwise reproducible run to run.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "edk_matvec.h"
+
+// ---------------------------------------------------------------------------
+// generic kernel: one lane per row, any basis kind, any local terms
+// ---------------------------------------------------------------------------
+template <typename VT, typename ST>
+__global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *__restrict__ x, VT *__restrict__ y,
+                                                        int64_t nb, int64_t col, int accumulate) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.D) return;
+  const TermsDev &T = A.T;
+  uint64_t fm = (1ull << A.w) - 1;
+  uint64_t s = A.kind == EDK_TENSOR ? edk_content_to_packed_d((uint64_t)i, A.L, A.base, A.w) : A.S.keys[i];
+  ST dr = 0, di = 0, orr = 0, oi = 0;
+  for (int t = 0; t < T.nterms; t++) {
+    const int *td = T.tdesc + t * EDK_TDESC;
+    int ns = td[0], m = td[1];
+    int loc = 0;
+    for (int q = 0; q < ns; q++) loc = loc * A.base + (int)((s >> td[2 + q]) & fm);
+    const int *rp = T.rowptr + T.mrow_off[m];
+    int e0 = rp[loc], e1 = rp[loc + 1];
+    for (int e = e0; e < e1; e++) {
+      int c = T.col[e];
+      double2 v = T.val[e];
+      if (c == loc) {
+        dr += (ST)v.x;
+        di += (ST)v.y;
+        continue;
+      }
+      uint64_t s2 = s;
+      int64_t delta = 0;
+      int cc = c;
+      for (int q = ns - 1; q >= 0; q--) {
+        uint64_t nd = (uint64_t)(cc % A.base);
+        cc /= A.base;
+        int sh = td[2 + q];
+        uint64_t od = (s2 >> sh) & fm;
+        s2 = (s2 & ~(fm << sh)) | (nd << sh);
+        delta += ((int64_t)nd - (int64_t)od) * (int64_t)T.tpw[t * EDK_MAXSITES + q];
+      }
+      int64_t j;
+      double cr = 1.0, ci = 0.0;
+      if (A.kind == EDK_TENSOR) {
+        j = i + delta;
+      } else if (A.kind == EDK_PROJECTED) {
+        j = edk_search(A.S, s2);
+        if (j < 0) continue;
+      } else {
+        CanonOut o = edk_canon(A.C, s2);
+        if (A.orbit_norms && !o.ok) continue;
+        j = edk_search(A.S, o.rep);
+        if (j < 0) continue;
+        double Rj = A.orbit_norms ? o.R : A.R[j];
+        cr = o.phase.x * Rj;   // conj(phase) * R_j
+        ci = -o.phase.y * Rj;
+      }
+      VT xj = x[j * nb + col];
+      // (v * coef) * xj
+      ST ar = (ST)(v.x * cr - v.y * ci), ai = (ST)(v.x * ci + v.y * cr);
+      orr += ar * (ST)xj.x - ai * (ST)xj.y;
+      oi += ar * (ST)xj.y + ai * (ST)xj.x;
+    }
+  }
+  VT xi = x[i * nb + col];
+  ST invR = (A.kind == EDK_TENSOR || A.kind == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.R[i]);
+  ST yr = dr * (ST)xi.x - di * (ST)xi.y + orr * invR;
+  ST yi = dr * (ST)xi.y + di * (ST)xi.x + oi * invR;
+  VT out;
+  if (accumulate) {
+    VT y0 = y[i * nb + col];
+    out.x = y0.x + yr;
+    out.y = y0.y + yi;
+  } else {
+    out.x = yr;
+    out.y = yi;
+  }
+  y[i * nb + col] = out;
+}
+
+// count of off-diagonal targets present in the sector (the NNZ_off figure)
+__global__ void __launch_bounds__(256) k_count_offdiag(MatvecArgs A, unsigned long long *count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n = 0;
+  if (i < A.D) {
+    const TermsDev &T = A.T;
+    uint64_t fm = (1ull << A.w) - 1;
+    uint64_t s = A.kind == EDK_TENSOR ? edk_content_to_packed_d((uint64_t)i, A.L, A.base, A.w) : A.S.keys[i];
+    for (int t = 0; t < T.nterms; t++) {
+      const int *td = T.tdesc + t * EDK_TDESC;
+      int ns = td[0], m = td[1];
+      int loc = 0;
+      for (int q = 0; q < ns; q++) loc = loc * A.base + (int)((s >> td[2 + q]) & fm);
+      const int *rp = T.rowptr + T.mrow_off[m];
+      for (int e = rp[loc]; e < rp[loc + 1]; e++) {
+        int c = T.col[e];
+        if (c == loc) continue;
+        uint64_t s2 = s;
+        int cc = c;
+        for (int q = ns - 1; q >= 0; q--) {
+          uint64_t nd = (uint64_t)(cc % A.base);
+          cc /= A.base;
+          int sh = td[2 + q];
+          s2 = (s2 & ~(fm << sh)) | (nd << sh);
+        }
+        if (A.kind == EDK_TENSOR) n++;
+        else if (A.kind == EDK_PROJECTED) n += edk_search(A.S, s2) >= 0;
+        else {
+          CanonOut o = edk_canon(A.C, s2);
+          n += edk_search(A.S, o.rep) >= 0;
+        }
+      }
+    }
+  }
+  // wave reduction then one atomic per wave (integer: order-independent)
+  for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o, 64);
+  if ((threadIdx.x & 63) == 0 && n) atomicAdd(count, n);
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+MatvecArgs edk_op::args() const {
+  MatvecArgs a;
+  a.C = basis->canon();
+  a.S = basis->search();
+  a.T = terms;
+  a.R = basis->d_R;
+  a.kind = basis->kind;
+  a.D = basis->D;
+  a.L = basis->L;
+  a.base = basis->base;
+  a.w = basis->w;
+  a.orbit_norms = basis->uploaded ? 0 : 1;
+  return a;
+}
+
+extern "C" int edk_operator_create(edk_ctx *ctx, edk_basis *b, int64_t nterms, const int64_t *site_off,
+                                   const int64_t *sites, const int64_t *colptr_off, const int64_t *colptr,
+                                   const int64_t *nz_off, const int64_t *rowval, const double *nzval,
+                                   edk_op **out) {
+  if (!ctx || !b || !out || nterms < 0) return edk_fail(EDK_ERR_ARG, "edk_operator_create: bad argument");
+  if (nterms > 0 && (!site_off || !sites || !colptr_off || !colptr || !nz_off))
+    return edk_fail(EDK_ERR_ARG, "edk_operator_create: null term arrays");
+  int L = b->L, base = b->base, w = b->w;
+  // deduplicate local matrices (trans_inv_operator repeats one matrix L times)
+  struct Mat {
+    int n;
+    std::vector<int64_t> cp, rv;
+    std::vector<double> nz;
+  };
+  std::vector<Mat> mats;
+  std::vector<int> tdesc(std::max<int64_t>(nterms, 1) * EDK_TDESC, 0);
+  std::vector<uint64_t> tpw(std::max<int64_t>(nterms, 1) * EDK_MAXSITES, 0);
+  bool all_real = true;
+  for (int64_t t = 0; t < nterms; t++) {
+    int64_t ns = site_off[t + 1] - site_off[t];
+    if (ns < 1 || ns > EDK_MAXSITES) return edk_fail(EDK_ERR_UNSUPPORTED, "local terms act on 1..8 sites");
+    int64_t n = 1;
+    for (int q = 0; q < ns; q++) n *= base;
+    if (n > 4096) return edk_fail(EDK_ERR_UNSUPPORTED, "local term dimension above 4096");
+    Mat M;
+    M.n = (int)n;
+    M.cp.assign(colptr + colptr_off[t], colptr + colptr_off[t] + n + 1);
+    int64_t nnz = nz_off[t + 1] - nz_off[t];
+    if (M.cp[0] != 1 || M.cp[n] - 1 != nnz) return edk_fail(EDK_ERR_ARG, "inconsistent colptr");
+    M.rv.assign(rowval + nz_off[t], rowval + nz_off[t] + nnz);
+    M.nz.assign(nzval + 2 * nz_off[t], nzval + 2 * (nz_off[t] + nnz));
+    for (int64_t q = 0; q < nnz; q++) {
+      if (M.rv[q] < 1 || M.rv[q] > n) return edk_fail(EDK_ERR_ARG, "rowval out of range");
+      if (M.nz[2 * q + 1] != 0.0) all_real = false;
+    }
+    int id = -1;
+    for (size_t m = 0; m < mats.size() && id < 0; m++)
+      if (mats[m].n == M.n && mats[m].cp == M.cp && mats[m].rv == M.rv && mats[m].nz == M.nz) id = (int)m;
+    if (id < 0) {
+      id = (int)mats.size();
+      mats.push_back(M);
+    }
+    tdesc[t * EDK_TDESC + 0] = (int)ns;
+    tdesc[t * EDK_TDESC + 1] = id;
+    for (int q = 0; q < ns; q++) {
+      int64_t site = sites[site_off[t] + q];
+      if (site < 1 || site > L) return edk_fail(EDK_ERR_ARG, "site index out of 1:L");
+      tdesc[t * EDK_TDESC + 2 + q] = (int)((L - site) * w);
+      tpw[t * EDK_MAXSITES + q] = b->pw[L - site];
+    }
+  }
+  // CSR of every distinct matrix: row r lists (c, M[r, c])
+  std::vector<int> mrow_off, rowptr, col;
+  std::vector<double2> val;
+  for (auto &M : mats) {
+    mrow_off.push_back((int)rowptr.size());
+    std::vector<std::vector<std::pair<int, double2>>> rows(M.n);
+    for (int c = 0; c < M.n; c++)
+      for (int64_t q = M.cp[c] - 1; q < M.cp[c + 1] - 1; q++)
+        rows[M.rv[q] - 1].push_back({c, make_double2(M.nz[2 * q], M.nz[2 * q + 1])});
+    for (int r = 0; r < M.n; r++) {
+      rowptr.push_back((int)col.size());
+      for (auto &e : rows[r]) {
+        col.push_back(e.first);
+        val.push_back(e.second);
+      }
+    }
+    rowptr.push_back((int)col.size());
+  }
+  if (mats.empty()) {
+    mrow_off.push_back(0);
+    rowptr.push_back(0);
+  }
+  if (col.empty()) {
+    col.push_back(0);
+    val.push_back(make_double2(0, 0));
+  }
+  edk_op *op = new edk_op();
+  op->ctx = ctx;
+  op->basis = b;
+  op->all_real = all_real;
+  op->nterms = (int)nterms;
+  // EDK_GENERIC_MATVEC=1 forces the generic kernel (lets tests compare kernels)
+  const char *gen = getenv("EDK_GENERIC_MATVEC");
+  op->disable_fast = gen && gen[0] == '1';
+  op->h_tdesc = tdesc;
+  op->h_mrow_off = mrow_off;
+  op->h_rowptr = rowptr;
+  op->h_col = col;
+  op->h_val = val;
+  int rc = EDK_OK;
+  EDK_HIP(hipSetDevice(ctx->device));
+  auto up = [&](const void *h, size_t bytes, void **d) -> int {
+    if (hipMalloc(d, std::max<size_t>(bytes, 16)) != hipSuccess) return edk_fail(EDK_ERR_OOM, "operator tables");
+    if (bytes && hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return edk_fail(EDK_ERR_HIP, "operator upload");
+    return EDK_OK;
+  };
+  if (!rc) rc = up(tdesc.data(), tdesc.size() * sizeof(int), (void **)&op->d_tdesc);
+  if (!rc) rc = up(tpw.data(), tpw.size() * sizeof(uint64_t), (void **)&op->d_tpw);
+  if (!rc) rc = up(mrow_off.data(), mrow_off.size() * sizeof(int), (void **)&op->d_mrow_off);
+  if (!rc) rc = up(rowptr.data(), rowptr.size() * sizeof(int), (void **)&op->d_rowptr);
+  if (!rc) rc = up(col.data(), col.size() * sizeof(int), (void **)&op->d_col);
+  if (!rc) rc = up(val.data(), val.size() * sizeof(double2), (void **)&op->d_val);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = edk_fail(EDK_ERR_HIP, "operator upload sync");
+  if (rc) {
+    edk_operator_destroy(op);
+    return rc;
+  }
+  op->terms.nterms = (int)nterms;
+  op->terms.tdesc = op->d_tdesc;
+  op->terms.tpw = op->d_tpw;
+  op->terms.mrow_off = op->d_mrow_off;
+  op->terms.rowptr = op->d_rowptr;
+  op->terms.col = op->d_col;
+  op->terms.val = op->d_val;
+  b->refcount++;
+  *out = op;
+  return EDK_OK;
+}
+
+extern "C" int edk_operator_destroy(edk_op *op) {
+  if (!op) return EDK_OK;
+  hipSetDevice(op->ctx->device);
+  hipStreamSynchronize(op->ctx->stream);
+  hipFree(op->d_tdesc);
+  hipFree(op->d_tpw);
+  hipFree(op->d_mrow_off);
+  hipFree(op->d_rowptr);
+  hipFree(op->d_col);
+  hipFree(op->d_val);
+  hipFree(op->d_scratch);
+  edk_fast_release(op);
+  if (op->basis) op->basis->refcount--;
+  delete op;
+  return EDK_OK;
+}
+
+extern "C" int edk_operator_count_offdiag(edk_op *op, int64_t *nnz) {
+  if (!op || !nnz) return edk_fail(EDK_ERR_ARG, "null argument");
+  edk_ctx *ctx = op->ctx;
+  EDK_HIP(hipSetDevice(ctx->device));
+  unsigned long long *d = nullptr, h = 0;
+  EDK_HIP(hipMalloc((void **)&d, 8));
+  EDK_HIP(hipMemsetAsync(d, 0, 8, ctx->stream));
+  MatvecArgs a = op->args();
+  if (a.D > 0) k_count_offdiag<<<(unsigned)((a.D + 255) / 256), 256, 0, ctx->stream>>>(a, d);
+  EDK_HIP(hipGetLastError());
+  EDK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  EDK_HIP(hipStreamSynchronize(ctx->stream));
+  hipFree(d);
+  *nnz = (int64_t)h;
+  return EDK_OK;
+}
+
+int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
+  edk_ctx *ctx = op->ctx;
+  MatvecArgs a = op->args();
+  if (a.D == 0) return EDK_OK;
+  unsigned grid = (unsigned)((a.D + 255) / 256);
+  if (edk_matvec_fast(op, dtype, y, x, nb, accumulate)) return EDK_OK;
+  for (int64_t c = 0; c < nb; c++) {
+    if (dtype == EDK_C64)
+      k_matvec_generic<double2, double><<<grid, 256, 0, ctx->stream>>>(a, (const double2 *)x, (double2 *)y, nb, c,
+                                                                       accumulate);
+    else
+      k_matvec_generic<float2, float><<<grid, 256, 0, ctx->stream>>>(a, (const float2 *)x, (float2 *)y, nb, c,
+                                                                     accumulate);
+  }
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+
+extern "C" int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
+  if (!op || !y || !x || nb < 1) return edk_fail(EDK_ERR_ARG, "edk_matvec: bad argument");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "edk_matvec: dtype must be EDK_C64 or EDK_C32");
+  if (x == y) return edk_fail(EDK_ERR_ARG, "edk_matvec: x and y must not alias");
+  EDK_HIP(hipSetDevice(op->ctx->device));
+  return edk_matvec_launch(op, dtype, y, x, nb, accumulate);
+}
+
+extern "C" int edk_matvec_host(edk_op *op, int dtype, void *yh, const void *xh, int64_t nb, int accumulate) {
+  if (!op || !yh || !xh || nb < 1) return edk_fail(EDK_ERR_ARG, "edk_matvec_host: bad argument");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "edk_matvec_host: bad dtype");
+  edk_ctx *ctx = op->ctx;
+  size_t bytes = (size_t)op->basis->D * nb * (dtype == EDK_C64 ? 16 : 8);
+  void *dx = nullptr, *dy = nullptr;
+  EDK_HIP(hipSetDevice(ctx->device));
+  if (hipMalloc(&dx, std::max<size_t>(bytes, 16)) != hipSuccess) return edk_fail(EDK_ERR_OOM, "matvec_host x");
+  if (hipMalloc(&dy, std::max<size_t>(bytes, 16)) != hipSuccess) {
+    hipFree(dx);
+    return edk_fail(EDK_ERR_OOM, "matvec_host y");
+  }
+  int rc = EDK_OK;
+  if (hipMemcpyAsync(dx, xh, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(dy, yh, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = edk_fail(EDK_ERR_HIP, "matvec_host copy in");
+  if (!rc) rc = edk_matvec_launch(op, dtype, dy, dx, nb, accumulate);
+  if (!rc && hipMemcpyAsync(yh, dy, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    rc = edk_fail(EDK_ERR_HIP, "matvec_host copy out");
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = edk_fail(EDK_ERR_HIP, "matvec_host sync");
+  hipFree(dx);
+  hipFree(dy);
+  return rc;
+}

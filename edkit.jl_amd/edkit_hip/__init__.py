@@ -1,0 +1,17 @@
+"""edkit_hip -- MI355X (gfx950) engine behind EDKit.jl's matrix-free Operator
+path, driven from Python through the C ABI of libedkit_hip.so.
+
+The names mirror the Julia API: bases (TensorBasis, ProjectedBasis,
+TranslationalBasis, TranslationParityBasis, basis), spin, operator,
+trans_inv_operator, mul_ (mul!), H @ x / H * x, mul, timeevolve,
+KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, plus the new
+lanczos_ground.
+"""
+from ._lib import EdkError, lib  # noqa: F401
+from .basis import (AbelianBasis, AbstractBasis, ProjectedBasis, TensorBasis,  # noqa: F401
+                    TranslationalBasis, TranslationParityBasis, basis, productstate)
+from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401
+from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
+from .matvec import apply, device_operator, matvec_into, mul, mul_  # noqa: F401
+from .operator import Operator, find_base, operator, trans_inv_operator  # noqa: F401
+from .spin import spin  # noqa: F401

@@ -1,0 +1,121 @@
+"""Loader of libedkit_hip.so (the HIP engine) with its C ABI signatures.
+
+The shared library is built in-tree next to this file.  There is no CPU
+fallback: if the library is missing or fails to load, every entry point
+raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libedkit_hip.so")
+
+EDK_OK = 0
+EDK_TENSOR, EDK_PROJECTED, EDK_TRANSLATIONAL, EDK_TRANSLATION_PARITY, EDK_ABELIAN = range(5)
+EDK_C64, EDK_C32 = 0, 1
+
+# every symbol include/edkit_hip.h declares
+SYMBOLS = [
+    "edk_last_error", "edk_version",
+    "edk_ctx_create", "edk_ctx_destroy", "edk_ctx_synchronize", "edk_ctx_stream",
+    "edk_malloc", "edk_free", "edk_memcpy_htod", "edk_memcpy_dtoh", "edk_memset",
+    "edk_basis_build", "edk_basis_upload", "edk_basis_dim", "edk_basis_order",
+    "edk_basis_download", "edk_basis_index", "edk_basis_destroy",
+    "edk_operator_create", "edk_operator_destroy", "edk_operator_count_offdiag",
+    "edk_matvec", "edk_matvec_host",
+    "edk_lanczos_ground",
+    "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
+    "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
+]
+
+
+class EdkError(RuntimeError):
+    """Error raised from a nonzero status of the C ABI."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[edk status {code}] {msg}")
+        self.code = code
+
+
+class BasisSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("L", C.c_int32), ("base", C.c_int32), ("a", C.c_int32),
+                ("N", C.c_int32), ("k", C.c_int32), ("p", C.c_int32), ("z", C.c_int32),
+                ("has_k", C.c_int32), ("has_p", C.c_int32), ("has_z", C.c_int32),
+                ("ncustom", C.c_int32), ("perms", C.c_void_p), ("ks", C.c_void_p),
+                ("invs", C.c_void_p)]
+
+
+class KrylovParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("m_init", C.c_int32), ("m_max", C.c_int32),
+                ("extend_step", C.c_int32), ("nsample", C.c_int32), ("bisect_iters", C.c_int32),
+                ("extend_basis", C.c_int32), ("normalize_output", C.c_int32)]
+
+
+class KrylovDiag(C.Structure):
+    _fields_ = [("basis_builds", C.c_int64), ("basis_extensions", C.c_int64),
+                ("restarts", C.c_int64), ("matvecs", C.c_int64),
+                ("total_times_served", C.c_int64), ("max_dim_used", C.c_int64),
+                ("n_accepted_intervals", C.c_int64)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load the library once and declare the signatures."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libedkit_hip.so not found at {LIB_PATH}: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D, SZ = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_size_t
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "edk_last_error": (C.c_char_p, []),
+        "edk_version": (I32, []),
+        "edk_ctx_create": (I32, [I32, P, PP]),
+        "edk_ctx_destroy": (I32, [P]),
+        "edk_ctx_synchronize": (I32, [P]),
+        "edk_ctx_stream": (P, [P]),
+        "edk_malloc": (I32, [P, SZ, PP]),
+        "edk_free": (I32, [P, P]),
+        "edk_memcpy_htod": (I32, [P, P, P, SZ]),
+        "edk_memcpy_dtoh": (I32, [P, P, P, SZ]),
+        "edk_memset": (I32, [P, P, I32, SZ]),
+        "edk_basis_build": (I32, [P, C.POINTER(BasisSpec), PP]),
+        "edk_basis_upload": (I32, [P, C.POINTER(BasisSpec), P, P, I64, PP]),
+        "edk_basis_dim": (I64, [P]),
+        "edk_basis_order": (I32, [P]),
+        "edk_basis_download": (I32, [P, P, P]),
+        "edk_basis_index": (I32, [P, P, I64, P, P, P]),
+        "edk_basis_destroy": (I32, [P]),
+        "edk_operator_create": (I32, [P, P, I64, P, P, P, P, P, P, P, PP]),
+        "edk_operator_destroy": (I32, [P]),
+        "edk_operator_count_offdiag": (I32, [P, C.POINTER(C.c_int64)]),
+        "edk_matvec": (I32, [P, I32, P, P, I64, I32]),
+        "edk_matvec_host": (I32, [P, I32, P, P, I64, I32]),
+        "edk_lanczos_ground": (I32, [P, I32, P, I32, D, I32, C.POINTER(D), C.POINTER(D),
+                                     C.POINTER(C.c_int), P]),
+        "edk_krylov_default_params": (None, [C.POINTER(KrylovParams)]),
+        "edk_krylov_create": (I32, [P, I32, P, C.POINTER(KrylovParams), PP]),
+        "edk_krylov_evolve": (I32, [P, D, P]),
+        "edk_krylov_diagnostics": (I32, [P, C.POINTER(KrylovDiag)]),
+        "edk_krylov_accepted_intervals": (I32, [P, P, I64, C.POINTER(C.c_int64)]),
+        "edk_krylov_destroy": (I32, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc):
+    """Raise EdkError for a nonzero status."""
+    if rc != EDK_OK:
+        raise EdkError(rc, lib().edk_last_error().decode())
+    return rc

@@ -1,0 +1,106 @@
+"""mul!(y, H, x), H*x and mul(H, x) on the device (Operator.jl:309-394).
+
+Host numpy vectors go through edk_matvec_host (copy in, gather kernel, copy
+out); device vectors (DeviceArray or a tensor on the device) go straight to
+edk_matvec.  ComplexF64 vectors run the FP64 kernel, ComplexF32 vectors the
+FP32 kernel.  A 2-D (D, nb) array is a block of nb right-hand sides.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+from .device import DeviceArray, dev_ptr, dtype_code
+
+
+class DeviceOperator:
+    """edk_op handle of an Operator on its basis' device."""
+
+    def __init__(self, H):
+        self.H = H
+        self.B = H.B
+        flat = H.flatten()
+        self._flat = flat
+        h = C.c_void_p()
+        n, so, si, co, cp, no, rv, nz = flat
+        check(lib().edk_operator_create(self.B.ctx.h, self.B.h, n, so.ctypes.data, si.ctypes.data,
+                                        co.ctypes.data, cp.ctypes.data, no.ctypes.data,
+                                        rv.ctypes.data, nz.ctypes.data, C.byref(h)))
+        self.h = h
+
+    def count_offdiag(self):
+        n = C.c_int64()
+        check(lib().edk_operator_count_offdiag(self.h, C.byref(n)))
+        return n.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                lib().edk_operator_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+
+def device_operator(H):
+    """The cached device handle of H (built on first use)."""
+    d = H._dev.get("op")
+    if d is None:
+        d = DeviceOperator(H)
+        H._dev["op"] = d
+    return d
+
+
+def _nb(shape, D):
+    if shape[0] != D:
+        raise ValueError(f"vector length {shape[0]} does not match the basis dimension {D}")
+    return 1 if len(shape) == 1 else int(np.prod(shape[1:]))
+
+
+def mul_(target, H, v):
+    """mul!(target, H, v): target += H v (Operator.jl:309-328)."""
+    op = device_operator(H)
+    D = H.B.D
+    if isinstance(v, np.ndarray):
+        if not isinstance(target, np.ndarray):
+            raise TypeError("target and v must both be host or both be device arrays")
+        dt = np.complex64 if v.dtype == np.complex64 and target.dtype == np.complex64 else np.complex128
+        x = np.ascontiguousarray(v, dt)
+        y = np.ascontiguousarray(target, dt)
+        nb = _nb(x.shape, D)
+        check(lib().edk_matvec_host(op.h, dtype_code(dt), y.ctypes.data, x.ctypes.data, nb, 1))
+        if y is not target:
+            target[...] = y
+        return target
+    nb = _nb(v.shape, D) if hasattr(v, "shape") else 1
+    check(lib().edk_matvec(op.h, dtype_code(v.dtype), dev_ptr(target), dev_ptr(v), nb, 1))
+    return target
+
+
+def apply(H, v):
+    """H * v with a zero-initialised target of the promoted type (Operator.jl:372-394)."""
+    op = device_operator(H)
+    D = H.B.D
+    if isinstance(v, np.ndarray):
+        dt = np.complex64 if v.dtype == np.complex64 else np.complex128
+        x = np.ascontiguousarray(v, dt)
+        y = np.zeros(x.shape, dt)
+        check(lib().edk_matvec_host(op.h, dtype_code(dt), y.ctypes.data, x.ctypes.data,
+                                    _nb(x.shape, D), 0))
+        return y
+    out = DeviceArray(v.shape, v.dtype, ctx=H.B.ctx, zero=False)
+    check(lib().edk_matvec(op.h, dtype_code(v.dtype), out.ptr, dev_ptr(v), _nb(v.shape, D), 0))
+    return out
+
+
+def matvec_into(y, H, x, accumulate=False):
+    """Device y = (accumulate ? y : 0) + H x without allocation."""
+    op = device_operator(H)
+    nb = _nb(x.shape, H.B.D) if hasattr(x, "shape") else 1
+    check(lib().edk_matvec(op.h, dtype_code(x.dtype), dev_ptr(y), dev_ptr(x), nb, int(accumulate)))
+    return y
+
+
+def mul(H, v):
+    """EDKit.mul(H, v) (Operator.jl:340-370): same result as H * v."""
+    return apply(H, v)

@@ -1,0 +1,197 @@
+/*
+ * edkit_hip.h -- C ABI of libedkit_hip.so, the MI355X (gfx950) engine behind
+ * EDKit.jl's matrix-free Operator path.
+ *
+ * EDKit.jl has no FFI layer of its own: its boundary is Julia multiple
+ * dispatch.  Each entry point below names the Julia method it stands in for
+ * (file:line in the EDKit.jl v0.5.0 tree); a Julia binding calls these through
+ * `ccall` (the binding is given in INTEGRATION.md next to this header).
+ *
+ * Conventions
+ *   - Every function returns an int status (EDK_OK == 0); on failure the
+ *     message is in edk_last_error() (thread-local).  Nothing throws.
+ *   - Handles are opaque.  The library owns every device allocation it makes;
+ *     host arrays passed in are only borrowed for the duration of the call.
+ *   - Representative tables cross the boundary 1-based exactly as the
+ *     reference stores `b.I`; the device keeps its own packed 0-based form.
+ *   - Vectors passed to the matvec and Krylov calls are DEVICE pointers
+ *     (from edk_malloc or any HIP allocation on the context's device).  A
+ *     block of nb vectors is row-interleaved: element (i, c) at x[i*nb + c].
+ *   - Complex vectors are interleaved (re, im): EDK_C64 = 2 x double,
+ *     EDK_C32 = 2 x float.
+ *   - A context is not reentrant: use one context per host thread.
+ */
+#ifndef EDKIT_HIP_H
+#define EDKIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define EDK_OK 0
+#define EDK_ERR_ARG 1         /* invalid argument (reference: error()/@assert) */
+#define EDK_ERR_HIP 2         /* HIP runtime error */
+#define EDK_ERR_OOM 3         /* device allocation failed */
+#define EDK_ERR_UNSUPPORTED 4 /* valid in the reference, not on the device */
+#define EDK_ERR_STATE 5       /* call out of order / handle still in use */
+#define EDK_ERR_NUMERIC 6     /* e.g. zero-norm start vector, invalid basis at tau=0 */
+
+/* basis kinds */
+#define EDK_TENSOR 0
+#define EDK_PROJECTED 1
+#define EDK_TRANSLATIONAL 2
+#define EDK_TRANSLATION_PARITY 3
+#define EDK_ABELIAN 4
+
+/* vector element types */
+#define EDK_C64 0 /* ComplexF64 */
+#define EDK_C32 1 /* ComplexF32 (matvec in FP32; Krylov scalars stay FP64) */
+
+typedef struct edk_ctx edk_ctx;
+typedef struct edk_basis edk_basis;
+typedef struct edk_op edk_op;
+typedef struct edk_krylov edk_krylov;
+
+const char *edk_last_error(void);
+int edk_version(void);
+
+/* ---- context ----------------------------------------------------------- */
+/* One HIP stream per context.  stream == NULL creates a private stream;
+ * otherwise the caller's hipStream_t is used (e.g. torch's current stream). */
+int edk_ctx_create(int device, void *stream, edk_ctx **out);
+int edk_ctx_destroy(edk_ctx *ctx);
+int edk_ctx_synchronize(edk_ctx *ctx);
+void *edk_ctx_stream(edk_ctx *ctx);
+
+/* device memory helpers for callers without their own allocator */
+int edk_malloc(edk_ctx *ctx, size_t bytes, void **dptr);
+int edk_free(edk_ctx *ctx, void *dptr);
+int edk_memcpy_htod(edk_ctx *ctx, void *dst, const void *src, size_t bytes);
+int edk_memcpy_dtoh(edk_ctx *ctx, void *dst, const void *src, size_t bytes);
+int edk_memset(edk_ctx *ctx, void *dptr, int value, size_t bytes);
+
+/* ---- bases -------------------------------------------------------------- */
+/* Keyword arguments of the reference constructors.  N < 0 means `nothing`.
+ *   EDK_TENSOR              TensorBasis(L, base)            AbstractBasis.jl:234-238
+ *   EDK_PROJECTED           ProjectedBasis(L, N, base)      ProjectedBasis.jl:244-260
+ *   EDK_TRANSLATIONAL       TranslationalBasis(L, k, N, a, base)
+ *                                                           TranslationalBasis.jl:235-266
+ *   EDK_TRANSLATION_PARITY  TranslationParityBasis(L, k, p, N, a, base)
+ *                                                           TranslationalParityBasis.jl:141-175
+ *   EDK_ABELIAN             basis(L, N, k, a, p, z, symmetries, base)
+ *                                                           AbelianBasis.jl:771-847
+ * For EDK_ABELIAN has_k/has_p/has_z tell which keywords were given; custom
+ * generators come as ncustom 1-based permutations of length L (perms),
+ * momenta (ks) and optional spin-inversion flags (invs, ncustom*L bytes). */
+typedef struct {
+  int32_t kind;
+  int32_t L, base, a;
+  int32_t N;
+  int32_t k, p, z;
+  int32_t has_k, has_p, has_z;
+  int32_t ncustom;
+  const int32_t *perms;
+  const int32_t *ks;
+  const uint8_t *invs;
+} edk_basis_spec;
+
+/* Enumerate the sector on the device (orbit-minimum test + prefix-sum
+ * compaction).  Replaces the constructors listed above. */
+int edk_basis_build(edk_ctx *ctx, const edk_basis_spec *spec, edk_basis **out);
+/* Build from host tables the caller already has (for example a basis whose
+ * predicate `f` was applied on the host): I 1-based sorted, R norms.
+ * Replaces passing a prebuilt basis object into operator(...) Operator.jl:56. */
+int edk_basis_upload(edk_ctx *ctx, const edk_basis_spec *spec,
+                     const int64_t *I_1based, const double *R, int64_t D,
+                     edk_basis **out);
+int64_t edk_basis_dim(const edk_basis *b);          /* size(b, 1) AbstractBasis.jl:142 */
+int edk_basis_order(const edk_basis *b);            /* order(b) */
+/* Copy the tables back: I 1-based (b.I), R (b.R).  Either may be NULL. */
+int edk_basis_download(edk_basis *b, int64_t *I_1based, double *R);
+/* index(b, dgt) for n states given as 0-based contents (the reference's
+ * index(dgt) - 1): 1-based pos and coefficient; absent gives (0, pos 1).
+ * TranslationalBasis.jl:310-341, TranslationalParityBasis.jl:192-234,
+ * AbelianBasis.jl:669-693, ProjectedBasis.jl:52-57, AbstractBasis.jl:246. */
+int edk_basis_index(edk_basis *b, const int64_t *contents, int64_t n,
+                    int64_t *pos_1based, double *coef_re, double *coef_im);
+int edk_basis_destroy(edk_basis *b);
+
+/* ---- operators ------------------------------------------------------------ */
+/* Operator{Tv,Tb}(M::Vector{SparseMatrixCSC}, I::Vector{Vector{Int}}, B)
+ * (Operator.jl:24-28, built by operator(...) :56-78 / trans_inv_operator
+ * :102-108), flattened:
+ *   site_off[nterms+1], sites[]        1-based site lists, first site = MSB
+ *   colptr_off[nterms]                 start of term t's colptr in colptr[]
+ *   colptr[]                           SparseMatrixCSC.colptr, 1-based
+ *   nz_off[nterms+1], rowval[]         SparseMatrixCSC.rowval, 1-based
+ *   nzval[]                            complex interleaved (re, im)
+ * Precondition of the gather kernels: the operator commutes with the basis'
+ * symmetry group (as every reference sector test assumes). */
+int edk_operator_create(edk_ctx *ctx, edk_basis *b, int64_t nterms,
+                        const int64_t *site_off, const int64_t *sites,
+                        const int64_t *colptr_off, const int64_t *colptr,
+                        const int64_t *nz_off, const int64_t *rowval,
+                        const double *nzval, edk_op **out);
+int edk_operator_destroy(edk_op *op);
+/* Exact number of off-diagonal local nonzeros whose target lies in the
+ * sector, summed over rows (device counting pass). */
+int edk_operator_count_offdiag(edk_op *op, int64_t *nnz_off);
+
+/* y = (accumulate ? y : 0) + H x, on device pointers, nb row-interleaved
+ * columns.  accumulate=1 is mul!(y, H, x) (Operator.jl:309-328); accumulate=0
+ * is the zero-filled target of H*x / _apply! (:372-394, TimeEvolution.jl:237-241). */
+int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
+               int accumulate);
+/* The literal host-array mul!(y, H, x): copies in, applies, copies out. */
+int edk_matvec_host(edk_op *op, int dtype, void *y_host, const void *x_host,
+                    int64_t nb, int accumulate);
+
+/* ---- Krylov --------------------------------------------------------------- */
+/* Lanczos ground state (no counterpart in the reference; built from the
+ * _lanczos_build! recurrence TimeEvolution.jl:250-306 with full
+ * reorthogonalisation).  x0: device vector (dtype); m: max steps; tol: stop
+ * when |beta_j Q[j,1]| <= tol (tol <= 0 runs m steps); refine: 1 adds a final
+ * FP64 Rayleigh quotient/residual with the ComplexF64 matvec on the Ritz
+ * vector.  Outputs: E0, residual, steps; vec_out (device, dtype) optional. */
+int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m,
+                       double tol, int refine, double *E0, double *resid,
+                       int *steps, void *vec_out);
+
+/* KrylovEvolutionCache (TimeEvolution.jl:84-218), device-resident. */
+typedef struct {
+  double tol;          /* 1e-10 */
+  int32_t m_init;      /* 30 */
+  int32_t m_max;       /* 60 */
+  int32_t extend_step; /* 10 */
+  int32_t nsample;     /* 13 */
+  int32_t bisect_iters;/* 12 */
+  int32_t extend_basis;     /* 1 */
+  int32_t normalize_output; /* 0 */
+} edk_krylov_params;
+
+typedef struct {
+  int64_t basis_builds, basis_extensions, restarts, matvecs,
+      total_times_served, max_dim_used, n_accepted_intervals;
+} edk_krylov_diag;
+
+void edk_krylov_default_params(edk_krylov_params *p);
+/* KrylovEvolutionCache(H, psi0; kwargs...) TimeEvolution.jl:166-218 */
+int edk_krylov_create(edk_op *op, int dtype, const void *psi0,
+                      const edk_krylov_params *params, edk_krylov **out);
+/* timeevolve!(cache, t) TimeEvolution.jl:586-597; out: device vector */
+int edk_krylov_evolve(edk_krylov *c, double t, void *out);
+/* KrylovEvolutionDiagnostics TimeEvolution.jl:37-59 */
+int edk_krylov_diagnostics(edk_krylov *c, edk_krylov_diag *d);
+/* accepted_intervals (up to cap entries); returns count in *n */
+int edk_krylov_accepted_intervals(edk_krylov *c, double *out, int64_t cap,
+                                  int64_t *n);
+int edk_krylov_destroy(edk_krylov *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

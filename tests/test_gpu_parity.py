@@ -1,0 +1,238 @@
+"""GPU parity: libedkit_hip (through the C ABI) against the oracle on the same
+seeded inputs.  Tables, indices and dimensions bit-exact; ComplexF64 matvec
+within 1e-12 relative; ComplexF32 matvec within 2e-6 relative (FP32 rounding
+of ~30 terms per row); Krylov results within the reference's own test
+tolerances."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.models import blbq, hopping_pm, j1j2, rand_state, xxz
+
+pytestmark = pytest.mark.gpu
+
+OKIND = {"TensorBasis": "tensor", "ProjectedBasis": "projected",
+         "TranslationalBasis": "translational", "TranslationParityBasis": "translation_parity"}
+
+
+def make_pair(edk, kind, L, base=2, **kw):
+    """Device basis and oracle basis with the same keywords."""
+    if kind == "basis":
+        B = edk.basis(L=L, base=base, **kw)
+        okw = dict(N=kw.get("N"), k=kw.get("k") or 0, p=kw.get("p") or 0, z=kw.get("z") or 0,
+                   has_k="k" in kw, has_p="p" in kw, has_z="z" in kw,
+                   symmetries=kw.get("symmetries"), a=kw.get("a", 1))
+        ob = orc.Basis("abelian", L, base=base, **okw)
+        return B, ob
+    B = getattr(edk, kind)(L=L, base=base, **kw)
+    ob = orc.Basis(OKIND[kind], L, base=base, **kw)
+    return B, ob
+
+
+TABLE_CASES = [
+    ("TensorBasis", 10, 2, {}),
+    ("ProjectedBasis", 14, 2, dict(N=7)),
+    ("ProjectedBasis", 8, 3, dict(N=8)),
+    ("TranslationalBasis", 12, 2, dict(N=6, k=0)),
+    ("TranslationalBasis", 12, 2, dict(N=5, k=5)),
+    ("TranslationalBasis", 12, 2, dict(N=6, k=3, a=2)),
+    ("TranslationalBasis", 14, 2, dict(k=1)),
+    ("TranslationalBasis", 8, 3, dict(N=8, k=2)),
+    ("TranslationParityBasis", 12, 2, dict(N=6, k=0, p=1)),
+    ("TranslationParityBasis", 12, 2, dict(N=6, k=0, p=-1)),
+    ("TranslationParityBasis", 12, 2, dict(N=6, k=6, p=1)),
+    ("TranslationParityBasis", 12, 2, dict(N=6, k=6, p=-1)),
+    ("TranslationParityBasis", 8, 3, dict(N=8, k=0, p=1)),
+    ("basis", 12, 2, dict(N=6, k=0, p=1)),
+    ("basis", 12, 2, dict(N=6, k=3)),
+    ("basis", 12, 2, dict(N=6, k=2, z=-1)),
+    ("basis", 10, 2, dict(k=1, p=1)),
+    ("basis", 8, 3, dict(N=8, k=1)),
+    ("basis", 12, 2, dict(N=5, k=0)),
+]
+
+
+@pytest.mark.parametrize("kind,L,base,kw", TABLE_CASES)
+def test_tables_bit_exact(gpu, kind, L, base, kw):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, kind, L, base, **kw)
+    assert B.D == ob.D
+    I, R = B.tables()
+    oI, oR = ob.tables()
+    assert np.array_equal(I, oI)
+    assert np.array_equal(R.view(np.uint64), oR.view(np.uint64))
+
+
+@pytest.mark.parametrize("kind,L,base,kw", TABLE_CASES)
+def test_index_bit_exact(gpu, kind, L, base, kw):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, kind, L, base, **kw)
+    n = base ** L
+    states = np.arange(n, dtype=np.int64) if n <= 1 << 14 else \
+        np.random.default_rng(5).integers(0, n, 1 << 14)
+    coef, pos = B.index(states)
+    opos, ocoef = ob.index(states.astype(np.uint64))
+    assert np.array_equal(pos, opos)
+    assert np.array_equal(coef.real, ocoef.real) and np.array_equal(coef.imag, ocoef.imag)
+
+
+def test_large_tables_bit_exact(gpu):
+    import edkit_hip as edk
+    for kind, kw in (("TranslationalBasis", dict(N=12, k=0)),
+                     ("TranslationParityBasis", dict(N=12, k=0, p=1)),
+                     ("TranslationalBasis", dict(N=12, k=7))):
+        B, ob = make_pair(edk, kind, 24, **kw)
+        I, R = B.tables()
+        oI, oR = ob.tables()
+        assert np.array_equal(I, oI) and np.array_equal(R, oR)
+    B = edk.TranslationalBasis(L=24, N=12, k=0)
+    assert B.D == 112720
+
+
+MV_CASES = [
+    ("TensorBasis", 10, 2, {}, xxz),
+    ("ProjectedBasis", 14, 2, dict(N=7), j1j2),
+    ("TranslationalBasis", 16, 2, dict(N=8, k=0), j1j2),
+    ("TranslationalBasis", 16, 2, dict(N=8, k=3), j1j2),
+    ("TranslationalBasis", 14, 2, dict(N=6, k=5), hopping_pm),
+    ("TranslationalBasis", 12, 2, dict(N=6, k=1, a=2), xxz),
+    ("TranslationParityBasis", 16, 2, dict(N=8, k=0, p=1), xxz),
+    ("TranslationParityBasis", 16, 2, dict(N=8, k=8, p=-1), j1j2),
+    ("TranslationParityBasis", 8, 3, dict(N=8, k=0, p=1), blbq),
+    ("TranslationalBasis", 8, 3, dict(N=8, k=3), blbq),
+    ("basis", 14, 2, dict(N=7, k=0, p=-1), j1j2),
+    ("basis", 12, 2, dict(N=6, k=4, z=1), xxz),
+    ("basis", 12, 2, dict(N=6, k=5), j1j2),
+]
+
+
+@pytest.mark.parametrize("kind,L,base,kw,model", MV_CASES)
+def test_matvec_c64_matches_oracle(gpu, kind, L, base, kw, model):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, kind, L, base, **kw)
+    H = model(B)
+    x = rand_state(B.D, 17)
+    y = H @ x
+    yo = orc.Op(ob, H.flatten()) @ x
+    assert np.linalg.norm(y - yo) <= 1e-12 * np.linalg.norm(yo)
+    # mul! accumulates: y += H x
+    y2 = y.copy()
+    edk.mul_(y2, H, x)
+    assert np.linalg.norm(y2 - 2 * yo) <= 1e-12 * np.linalg.norm(yo)
+
+
+@pytest.mark.parametrize("kind,L,base,kw,model", MV_CASES[:8])
+def test_matvec_c32_matches_oracle(gpu, kind, L, base, kw, model):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, kind, L, base, **kw)
+    H = model(B)
+    x = rand_state(B.D, 18)
+    y = H @ x.astype(np.complex64)
+    assert y.dtype == np.complex64
+    yo = orc.Op(ob, H.flatten()) @ x.astype(np.complex64).astype(np.complex128)
+    assert np.linalg.norm(y - yo) <= 2e-6 * np.linalg.norm(yo)
+
+
+def test_matvec_block_and_device_vectors(gpu):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, "TranslationParityBasis", 16, N=8, k=0, p=1)
+    H = j1j2(B, 0.3)
+    X = np.stack([rand_state(B.D, s) for s in range(4)], axis=1)
+    Y = H @ X
+    op = orc.Op(ob, H.flatten())
+    for c in range(4):
+        yo = op @ X[:, c]
+        assert np.linalg.norm(Y[:, c] - yo) <= 1e-12 * np.linalg.norm(yo)
+    dx = edk.DeviceArray.from_host(X[:, 0].copy())
+    dy = H @ dx
+    assert np.linalg.norm(dy.to_host() - Y[:, 0]) <= 1e-14 * np.linalg.norm(Y[:, 0])
+
+
+def test_offdiag_count_matches_oracle(gpu):
+    import edkit_hip as edk
+    from edkit_hip.matvec import device_operator
+    B, ob = make_pair(edk, "TranslationParityBasis", 20, N=10, k=0, p=1)
+    H = xxz(B)
+    assert device_operator(H).count_offdiag() == orc.Op(ob, H.flatten()).count_offdiag()
+
+
+def test_full_size_l32_columns_and_hermiticity(gpu):
+    """L=32 TranslationParityBasis N=16 k=0 p=+1 (D = 9 398 520): dimension,
+    sortedness, single columns against the oracle's colmn!, Hermiticity."""
+    import edkit_hip as edk
+    B = edk.TranslationParityBasis(L=32, N=16, k=0, p=1)
+    assert B.D == 9398520
+    I, R = B.tables()
+    assert np.all(np.diff(I) > 0)
+    H = xxz(B, 1.0)
+    rng = np.random.default_rng(3)
+    cols = rng.integers(0, B.D, 4)
+    # the oracle's index() needs the table: build its basis once from scratch
+    ob = orc.Basis("translation_parity", 32, N=16, k=0, p=1)
+    oI, oR = ob.tables()
+    assert np.array_equal(I, oI) and np.array_equal(R, oR)
+    op = orc.Op(ob, H.flatten())
+    for j in cols:
+        e = edk.DeviceArray((B.D,), np.complex128)
+        ej = np.zeros(1, np.complex128) + 1
+        e_h = np.zeros(B.D, np.complex128)
+        e_h[j] = ej[0]
+        e.copy_from(e_h)
+        y = (H @ e).to_host()
+        rows, vals = op.column(j)
+        ref = np.zeros(B.D, np.complex128)
+        ref[rows] = vals
+        assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    u = edk.DeviceArray.from_host(rand_state(B.D, 1))
+    v = edk.DeviceArray.from_host(rand_state(B.D, 2))
+    Hu, Hv = (H @ u).to_host(), (H @ v).to_host()
+    a = np.vdot(u.to_host(), Hv)
+    b = np.vdot(v.to_host(), Hu)
+    assert abs(a - np.conj(b)) <= 1e-12 * max(abs(a), 1e-30) * 10
+
+
+def test_lanczos_matches_oracle(gpu):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, "TranslationalBasis", 16, N=8, k=0)
+    H = j1j2(B, 0.5)
+    v0 = rand_state(B.D, 4)
+    E_o, _, _, _ = orc.lanczos_ground(orc.Op(ob, H.flatten()), v0, 200, 1e-11)
+    E, res, steps, vec = edk.lanczos_ground(H, v0, m=200, tol=1e-11, want_vec=True)
+    assert abs(E - E_o) < 1e-10
+    assert res < 1e-8
+    y = vec.to_host()
+    r = H @ y - E * y
+    assert np.linalg.norm(r) < 1e-8
+    # precision form: FP32 vectors and matvec, FP64 Krylov scalars + FP64 refinement
+    E32, res32, _, _ = edk.lanczos_ground(H, v0.astype(np.complex64), m=200, tol=1e-9,
+                                          dtype=np.complex64, refine=True)
+    assert abs(E32 - E_o) < 1e-9
+
+
+def test_krylov_matches_oracle(gpu):
+    import edkit_hip as edk
+    B, ob = make_pair(edk, "TranslationalBasis", 14, N=7, k=0)
+    H = hopping_pm(B)
+    psi = rand_state(B.D, 11)
+    ts = [0.3, 0.9, 1.4, 4.0]
+    out_o, d_o = orc.timeevolve(orc.Op(ob, H.flatten()), psi, ts, tol=1e-11, m_init=20, m_max=40)
+    out, d = edk.timeevolve(H, psi, ts, return_diagnostics=True, tol=1e-11, m_init=20, m_max=40)
+    assert np.max(np.linalg.norm(out - out_o, axis=0)) < 1e-9
+    for key in ("basis_builds", "restarts", "total_times_served"):
+        assert d[key] == d_o[key]
+    out32 = edk.timeevolve(H, psi, ts, dtype=np.complex64, tol=1e-6, m_init=20, m_max=40)
+    assert np.max(np.linalg.norm(out32 - out_o, axis=0)) < 1e-4
+
+
+def test_errors_are_reported(gpu):
+    import edkit_hip as edk
+    with pytest.raises(edk.EdkError):
+        edk.TranslationParityBasis(L=12, N=6, k=1, p=1)
+    B = edk.TensorBasis(L=4)
+    H = xxz(B)
+    with pytest.raises(edk.EdkError):
+        edk.KrylovEvolutionCache(H, np.zeros(16, np.complex128))
+    c = edk.KrylovEvolutionCache(H, rand_state(16, 1), m_init=4, m_max=8)
+    c.evolve(1.0)
+    with pytest.raises(edk.EdkError):
+        c.evolve(0.5)
