@@ -236,3 +236,23 @@ def test_errors_are_reported(gpu):
     c.evolve(1.0)
     with pytest.raises(edk.EdkError):
         c.evolve(0.5)
+
+
+def test_fast_and_generic_kernels_agree(gpu, monkeypatch):
+    """The specialised two-site-family kernel against the generic kernel."""
+    import edkit_hip as edk
+    cases = [("TranslationParityBasis", dict(N=9, k=0, p=1), 18, xxz),
+             ("TranslationParityBasis", dict(N=9, k=9, p=-1), 18, j1j2),
+             ("TranslationalBasis", dict(N=9, k=4), 18, j1j2),
+             ("TranslationalBasis", dict(N=8, k=3, a=2), 16, xxz),
+             ("ProjectedBasis", dict(N=8), 16, j1j2)]
+    for kind, kw, L, model in cases:
+        B = getattr(edk, kind)(L=L, **kw)
+        x = rand_state(B.D, 9)
+        for dt in (np.complex128, np.complex64):
+            monkeypatch.setenv("EDK_GENERIC_MATVEC", "0")
+            yf = model(B) @ x.astype(dt)
+            monkeypatch.setenv("EDK_GENERIC_MATVEC", "1")
+            yg = model(B) @ x.astype(dt)
+            tol = 1e-13 if dt == np.complex128 else 1e-6
+            assert np.linalg.norm(yf - yg) <= tol * np.linalg.norm(yg), (kind, kw, dt)
