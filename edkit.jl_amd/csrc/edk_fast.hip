@@ -20,13 +20,13 @@
 // together, so each lane keeps eight independent memory requests in flight.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "edk_matvec.h"
 
 #define FAST_MAXFAM 4
 #define FAST_MAXENT 12
-#define FAST_BATCH 8
 #define FAST_TPB 256
 
 struct FamEnt {
@@ -51,15 +51,17 @@ struct FastArgs {
   int kind, L, A, len, kp, P;
   uint32_t mask;
   int ndiv;
-  int divs[16];  // divisors of len, ascending (orbit period candidates)
+  int divs[16];  // divisors of len below len, ascending (orbit period candidates)
+  int batch;     // targets per lane handled together (4, 8 or 16)
+  int scan;      // bucket width searched with one wide scan (0: binary search only)
   const FastTab *tab;
   const double *Rtab;  // orbit norms (same table the enumerator used)
   const double2 *ph;   // TranslationalBasis phases C[m]
-  const uint32_t *keys;
+  const uint32_t *keys;  // D keys + padding of 0xffffffff
   const uint32_t *dir;
   int dshift;
   uint32_t ndir;
-  int64_t D;
+  uint32_t D;
   const double *Rrow;
 };
 
@@ -72,7 +74,11 @@ __device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t m
 __device__ __forceinline__ uint32_t rev32(uint32_t s, int L) { return __brev(s) >> (32 - L); }
 
 // Canonicalise s'.  Returns false when the representative cannot be in the
-// table; otherwise rep and the real or complex factor conj(phase) * R_rep.
+// table; otherwise rep and the real (cr) or complex (cr, ci) factor
+// conj(phase) * R_rep.  All loops have uniform trip counts (no per-lane
+// divergence): the minimum over a whole cycle of len rotations equals the
+// minimum over one period, and the first index attaining it lies in the
+// first period, as in the reference's loops that stop at the period.
 template <int KIND, bool CPLX>
 __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
                                            double &ci) {
@@ -83,29 +89,27 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
     return true;
   }
   const int sh = A.A;
-  // orbit period: smallest divisor p of len with rot^p(s) == s
   int period = A.len;
   for (int q = 0; q < A.ndiv; q++) {
     int p = A.divs[q];
-    if (p < A.len && rotr32(s, p * sh, A.L, A.mask) == s) {
-      period = p;
-      break;
-    }
-  }
-  uint32_t minA = s, cur = s;
-  for (int i = 1; i < period; i++) {
-    cur = rotr32(cur, sh, A.L, A.mask);
-    minA = min(minA, cur);
+    bool m = rotr32(s, p * sh, A.L, A.mask) == s;
+    period = (m && period == A.len) ? p : period;
   }
   if (KIND == EDK_TRANSLATIONAL) {
+    uint32_t minA = s, cur = s;
+    for (int i = 1; i < A.len; i++) {
+      cur = rotr32(cur, sh, A.L, A.mask);
+      minA = min(minA, cur);
+    }
+    rep = minA;
     if (((int64_t)period * A.kp) % A.len) return false;  // check_momentum
     double R = A.Rtab[period - 1];
     if (CPLX || A.kp != 0) {
       int M = 0;
       cur = s;
-      while (cur != minA) {
+      for (int i = 1; i < A.len; i++) {
         cur = rotr32(cur, sh, A.L, A.mask);
-        M++;
+        M = (cur == minA && M == 0 && s != minA) ? i : M;
       }
       double2 c = A.ph[M];
       cr = c.x * R;
@@ -114,40 +118,32 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
       cr = R;
       ci = 0.0;
     }
-    rep = minA;
     return true;
   }
   // TranslationParityBasis
   uint32_t rb = rev32(s, A.L);
-  uint32_t minB = rb;
-  cur = rb;
-  for (int i = 1; i < period; i++) {
+  uint32_t minA = s, minB = rb, cur = s, curB = rb;
+  for (int i = 1; i < A.len; i++) {
     cur = rotr32(cur, sh, A.L, A.mask);
-    minB = min(minB, cur);
+    curB = rotr32(curB, sh, A.L, A.mask);
+    minA = min(minA, cur);
+    minB = min(minB, curB);
   }
   bool qp = (minA == minB);
   double R = A.Rtab[(qp ? A.len : 0) + period - 1];
   double sgn = 1.0;
   if (A.kp != 1 || A.P != 1) {
-    // index of the first minimum along the orbit; the reflected branch uses
-    // reverse(rot^M(s)) == rot^(-M)(reverse(s)) (TranslationalParityBasis.jl:222-231)
-    if (minB < minA) {
-      int M = 0;
-      cur = s;
-      while (rev32(cur, A.L) != minB) {
-        cur = rotr32(cur, sh, A.L, A.mask);
-        M++;
-      }
-      sgn = (double)A.P * ((A.kp == 1 || (M & 1) == 0) ? 1.0 : -1.0);
-    } else {
-      int M = 0;
-      cur = s;
-      while (cur != minA) {
-        cur = rotr32(cur, sh, A.L, A.mask);
-        M++;
-      }
-      sgn = (A.kp == 1 || (M & 1) == 0) ? 1.0 : -1.0;
+    // first index M with rot^M(s) (or reverse(rot^M(s))) at the minimum
+    // (TranslationalParityBasis.jl:207-231)
+    int Ma = -1, Mb = -1;
+    cur = s;
+    for (int i = 0; i < A.len; i++) {
+      Ma = (cur == minA && Ma < 0) ? i : Ma;
+      Mb = (rev32(cur, A.L) == minB && Mb < 0) ? i : Mb;
+      cur = rotr32(cur, sh, A.L, A.mask);
     }
+    if (minB < minA) sgn = (double)A.P * ((A.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (A.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
   }
   rep = minA < minB ? minA : minB;
   cr = sgn * R;
@@ -155,21 +151,131 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
   return true;
 }
 
-template <int KIND, bool CPLX, typename VT, typename ST>
-__global__ void __launch_bounds__(FAST_TPB) k_matvec_fast(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
-                                                          int accumulate) {
-  int64_t i = (int64_t)blockIdx.x * FAST_TPB + threadIdx.x;
+// The same canonicaliser for L = 32 and unit cell 1, fully unrolled: each of
+// the 31 rotations is one v_alignbit, the minima fold into v_min3, the
+// orbit period (a power of two dividing 32) is read off the same rotations.
+template <int KIND, bool CPLX>
+__device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
+                                             double &ci) {
+  if (KIND == EDK_PROJECTED) {
+    rep = s;
+    cr = 1.0;
+    ci = 0.0;
+    return true;
+  }
+  // rotations as a dependent chain (rotate by one each step): few live
+  // registers, so the batch of targets can be interleaved without spills
+  uint32_t minA = s, cur = s;
+#pragma unroll
+  for (int k = 1; k < 32; k++) {
+    cur = __builtin_amdgcn_alignbit(cur, cur, 1);
+    minA = min(minA, cur);
+  }
+  int period = 32;
+#pragma unroll
+  for (int k = 16; k >= 1; k >>= 1) period = (__builtin_amdgcn_alignbit(s, s, k) == s) ? k : period;
+  if (KIND == EDK_TRANSLATIONAL) {
+    rep = minA;
+    if (((int64_t)period * A.kp) % 32) return false;  // check_momentum
+    double R = A.Rtab[period - 1];
+    if (CPLX || A.kp != 0) {
+      int M = 0;
+      bool found = (s == minA);
+      cur = s;
+#pragma unroll
+      for (int k = 1; k < 32; k++) {
+        cur = __builtin_amdgcn_alignbit(cur, cur, 1);
+        M = (!found && cur == minA) ? k : M;
+        found = found || cur == minA;
+      }
+      double2 c = A.ph[M];
+      cr = c.x * R;
+      ci = -c.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    return true;
+  }
+  uint32_t rb = __brev(s);
+  uint32_t minB = rb;
+  cur = rb;
+#pragma unroll
+  for (int k = 1; k < 32; k++) {
+    cur = __builtin_amdgcn_alignbit(cur, cur, 1);
+    minB = min(minB, cur);
+  }
+  bool qp = (minA == minB);
+  double R = A.Rtab[(qp ? 32 : 0) + period - 1];
+  double sgn = 1.0;
+  if (A.kp != 1 || A.P != 1) {
+    // first k with rot^k(s) == minA, resp. reverse(rot^k(s)) == minB
+    // (TranslationalParityBasis.jl:207-231)
+    int Ma = 0, Mb = 0;
+    bool fa = (s == minA), fb = (rb == minB);
+    cur = s;
+#pragma unroll
+    for (int k = 1; k < 32; k++) {
+      cur = __builtin_amdgcn_alignbit(cur, cur, 1);
+      uint32_t cb = __brev(cur);
+      Ma = (!fa && cur == minA) ? k : Ma;
+      fa = fa || cur == minA;
+      Mb = (!fb && cb == minB) ? k : Mb;
+      fb = fb || cb == minB;
+    }
+    if (minB < minA) sgn = (double)A.P * ((A.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (A.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
+  }
+  rep = minA < minB ? minA : minB;
+  cr = sgn * R;
+  ci = 0.0;
+  return true;
+}
+
+// Position of key in the sorted table, or -1.  The directory bounds the
+// search to [lo, hi); a bucket of at most A.scan keys is searched with one
+// set of independent 16-byte loads (one memory round), a larger one by binary
+// search.
+__device__ __forceinline__ int64_t fast_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
+  if (hi <= lo) return -1;
+  if (hi - lo <= (uint32_t)A.scan) {
+    uint32_t base = lo & ~3u;
+    const uint4 *p = (const uint4 *)(A.keys + base);
+    int below = 0, eq = 0;
+    int nv = (int)((hi - base + 3) >> 2);
+#pragma unroll 4
+    for (int v = 0; v < nv; v++) {
+      uint4 w = p[v];
+      uint32_t idx = base + 4 * v;
+      below += (idx + 0 >= lo && idx + 0 < hi && w.x < key);
+      below += (idx + 1 >= lo && idx + 1 < hi && w.y < key);
+      below += (idx + 2 >= lo && idx + 2 < hi && w.z < key);
+      below += (idx + 3 >= lo && idx + 3 < hi && w.w < key);
+      eq += (idx + 0 >= lo && idx + 0 < hi && w.x == key);
+      eq += (idx + 1 >= lo && idx + 1 < hi && w.y == key);
+      eq += (idx + 2 >= lo && idx + 2 < hi && w.z == key);
+      eq += (idx + 3 >= lo && idx + 3 < hi && w.w == key);
+    }
+    return eq ? (int64_t)lo + below : -1;
+  }
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (A.keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < A.D && A.keys[lo] == key) ? (int64_t)lo : -1;
+}
+
+template <int KIND, bool CPLX, int NB, int LT, typename VT, typename ST>
+__global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 5 : 6))
+    k_matvec_fast(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y, int accumulate) {
+  uint32_t i = blockIdx.x * FAST_TPB + threadIdx.x;
   if (i >= A.D) return;
   const FastTab &T = *A.tab;
   const uint32_t s = A.keys[i];
   const uint32_t ns = ~s & A.mask;
   ST dg = 0;
   ST orr = 0, oi = 0;
-  // target generator state
-  int f = 0, e = 0;
-  uint32_t bits = 0;
-  uint32_t r = 0;
-  bool more = T.nfam > 0;
   // diagonal by popcounts
   for (int ff = 0; ff < T.nfam; ff++) {
     const FastFam &F = T.fam[ff];
@@ -178,19 +284,17 @@ __global__ void __launch_bounds__(FAST_TPB) k_matvec_fast(FastArgs A, const VT *
     dg += (ST)(F.diag[0] * __popc(ns & nr) + F.diag[1] * __popc(ns & rr) + F.diag[2] * __popc(s & nr) +
                F.diag[3] * __popc(s & rr));
   }
-  if (more) {
-    r = rotr32(s, A.L - T.fam[0].d, A.L, A.mask);
-    bits = 0;
-    e = -1;
-  }
+  // target generator: family f, entry e, remaining bit positions
+  int f = 0, e = -1;
+  uint32_t bits = 0;
+  uint32_t r = T.nfam > 0 ? rotr32(s, A.L - T.fam[0].d, A.L, A.mask) : 0;
+  bool more = T.nfam > 0;
   while (true) {
-    uint32_t tg[FAST_BATCH];
-    double tv[FAST_BATCH];
+    uint32_t tg[NB];
+    double tv[NB];
     int n = 0;
-    // collect up to FAST_BATCH targets
-    while (n < FAST_BATCH && more) {
+    while (n < NB && more) {
       if (bits == 0) {
-        // advance to the next entry / family
         e++;
         while (f < T.nfam && e >= T.fam[f].nent) {
           f++;
@@ -220,51 +324,42 @@ __global__ void __launch_bounds__(FAST_TPB) k_matvec_fast(FastArgs A, const VT *
       n++;
     }
     if (n == 0) break;
-    // canonicalise
-    uint32_t rep[FAST_BATCH];
-    double cr[FAST_BATCH], ci[FAST_BATCH];
-    int64_t lo[FAST_BATCH], hi[FAST_BATCH];
-    bool valid[FAST_BATCH];
+    uint32_t rep[NB], lo[NB], hi[NB];
+    ST cfr[NB], cfi[CPLX ? NB : 1];
 #pragma unroll
-    for (int q = 0; q < FAST_BATCH; q++) {
+    for (int q = 0; q < NB; q++) {
       lo[q] = 0;
       hi[q] = 0;
-      valid[q] = false;
+      rep[q] = 0;
+      double cr = 0.0, ci = 0.0;
+      bool okc = false;
       if (q < n) {
-        bool ok = fast_canon<KIND, CPLX>(A, tg[q], rep[q], cr[q], ci[q]);
-        if (ok) {
-          valid[q] = true;
-          uint32_t bkt = rep[q] >> A.dshift;
-          if (bkt < A.ndir) {
-            lo[q] = A.dir[bkt];
-            hi[q] = A.dir[bkt + 1];
-          }
+        if (LT == 32) okc = fast_canon32<KIND, CPLX>(A, tg[q], rep[q], cr, ci);
+        else okc = fast_canon<KIND, CPLX>(A, tg[q], rep[q], cr, ci);
+      }
+      if (okc) {
+        uint32_t bkt = rep[q] >> A.dshift;
+        if (bkt < A.ndir) {
+          lo[q] = A.dir[bkt];
+          hi[q] = A.dir[bkt + 1];
         }
       }
+      cfr[q] = (ST)(tv[q] * cr);
+      if (CPLX) cfi[q] = (ST)(tv[q] * ci);
     }
-    // in-bucket lower_bound, all targets advancing together
-    bool act = true;
-    while (act) {
-      act = false;
+    uint32_t jj[NB];
 #pragma unroll
-      for (int q = 0; q < FAST_BATCH; q++) {
-        if (lo[q] < hi[q]) {
-          int64_t mid = (lo[q] + hi[q]) >> 1;
-          if (A.keys[mid] < rep[q]) lo[q] = mid + 1;
-          else hi[q] = mid;
-          act = true;
-        }
-      }
-    }
+    for (int q = 0; q < NB; q++) jj[q] = (uint32_t)fast_find(A, rep[q], lo[q], hi[q]);
 #pragma unroll
-    for (int q = 0; q < FAST_BATCH; q++) {
-      if (valid[q] && lo[q] < A.D) {
-        int64_t j = lo[q];
-        if (A.keys[j] == rep[q]) {
-          VT xj = x[j];
-          ST ar = (ST)(tv[q] * cr[q]), ai = (ST)(tv[q] * ci[q]);
-          orr += ar * (ST)xj.x - ai * (ST)xj.y;
-          oi += ar * (ST)xj.y + ai * (ST)xj.x;
+    for (int q = 0; q < NB; q++) {
+      if (jj[q] != 0xffffffffu) {
+        VT xj = x[jj[q]];
+        if (CPLX) {
+          orr += cfr[q] * (ST)xj.x - cfi[q] * (ST)xj.y;
+          oi += cfr[q] * (ST)xj.y + cfi[q] * (ST)xj.x;
+        } else {
+          orr += cfr[q] * (ST)xj.x;
+          oi += cfr[q] * (ST)xj.y;
         }
       }
     }
@@ -415,16 +510,25 @@ static bool fast_setup(edk_op *op) {
   A.P = b->P;
   A.mask = b->L == 32 ? 0xffffffffu : ((1u << b->L) - 1);
   A.ndiv = 0;
-  for (int p = 1; p <= b->len && A.ndiv < 16; p++)
+  for (int p = 1; p < b->len && A.ndiv < 16; p++)
     if (b->len % p == 0) A.divs[A.ndiv++] = p;
+  // tuning knobs (defaults picked by measurement; the env vars override)
+  const char *kb = getenv("EDK_FAST_KEYS_PER_BUCKET");
+  const char *sc = getenv("EDK_FAST_SCAN");
+  int keys_per_bucket = kb ? atoi(kb) : 1;
+  A.scan = sc ? atoi(sc) : 0;
+  const char *bt = getenv("EDK_FAST_BATCH");
+  A.batch = bt ? atoi(bt) : 2;
   S->cplx = (b->kind == EDK_TRANSLATIONAL && !(b->kp == 0 || 2 * b->kp == b->len));
   edk_ctx *ctx = op->ctx;
   if (hipMalloc((void **)&S->d_tab, sizeof(FastTab)) != hipSuccess) return false;
   if (hipMemcpyAsync(S->d_tab, &tab, sizeof(FastTab), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return false;
-  if (hipMalloc((void **)&S->d_keys32, b->D * sizeof(uint32_t)) != hipSuccess) return false;
+  // keys + 64 padding entries (the wide scan may read past the end)
+  if (hipMalloc((void **)&S->d_keys32, (b->D + 64) * sizeof(uint32_t)) != hipSuccess) return false;
+  if (hipMemsetAsync(S->d_keys32 + b->D, 0xff, 64 * sizeof(uint32_t), ctx->stream) != hipSuccess) return false;
   k_keys32<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(b->d_keys, b->D, S->d_keys32);
-  // directory over the 32-bit keys: about one slot per 16 representatives
-  int64_t target = std::max<int64_t>(1024, b->D / 16);
+  // directory over the 32-bit keys: about one slot per keys_per_bucket representatives
+  int64_t target = std::max<int64_t>(1024, b->D / std::max(1, keys_per_bucket));
   int sh = 0;
   while (sh < 31 && (int64_t)(b->maxkey >> sh) + 1 > target) sh++;
   uint32_t ndir = (uint32_t)((b->maxkey >> sh) + 1);
@@ -438,19 +542,33 @@ static bool fast_setup(edk_op *op) {
   A.dir = S->d_dir;
   A.dshift = sh;
   A.ndir = ndir;
-  A.D = b->D;
+  A.D = (uint32_t)b->D;
   A.Rrow = b->d_R;
   S->ok = true;
   return true;
 }
 
-template <int KIND, bool CPLX>
-static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
+template <int KIND, bool CPLX, int NB, int LT>
+static void launch_fast_nb(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
   unsigned grid = (unsigned)((A.D + FAST_TPB - 1) / FAST_TPB);
   if (dtype == EDK_C64)
-    k_matvec_fast<KIND, CPLX, double2, double><<<grid, FAST_TPB, 0, st>>>(A, (const double2 *)x, (double2 *)y, acc);
+    k_matvec_fast<KIND, CPLX, NB, LT, double2, double>
+        <<<grid, FAST_TPB, 0, st>>>(A, (const double2 *)x, (double2 *)y, acc);
   else
-    k_matvec_fast<KIND, CPLX, float2, float><<<grid, FAST_TPB, 0, st>>>(A, (const float2 *)x, (float2 *)y, acc);
+    k_matvec_fast<KIND, CPLX, NB, LT, float2, float><<<grid, FAST_TPB, 0, st>>>(A, (const float2 *)x, (float2 *)y, acc);
+}
+
+template <int KIND, bool CPLX, int LT>
+static void launch_fast_lt(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
+  if (A.batch >= 8) launch_fast_nb<KIND, CPLX, 8, LT>(A, dtype, y, x, acc, st);
+  else if (A.batch >= 4) launch_fast_nb<KIND, CPLX, 4, LT>(A, dtype, y, x, acc, st);
+  else launch_fast_nb<KIND, CPLX, 2, LT>(A, dtype, y, x, acc, st);
+}
+
+template <int KIND, bool CPLX>
+static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
+  if (A.L == 32 && A.A == 1) launch_fast_lt<KIND, CPLX, 32>(A, dtype, y, x, acc, st);
+  else launch_fast_lt<KIND, CPLX, 0>(A, dtype, y, x, acc, st);
 }
 
 bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {

@@ -752,3 +752,52 @@ extern "C" int edk_krylov_destroy(edk_krylov *c) {
   krylov_free(c);
   return EDK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// device vector kernels of the C ABI
+// ---------------------------------------------------------------------------
+static int vec_work(edk_ctx *ctx, int dtype, int64_t n, KrylovWork &K) {
+  if (!ctx || n < 0) return edk_fail(EDK_ERR_ARG, "bad context or length");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "bad dtype");
+  EDK_HIP(hipSetDevice(ctx->device));
+  return K.init(ctx, n, dtype, 1);
+}
+
+extern "C" int edk_vec_scale(edk_ctx *ctx, int dtype, void *y, const void *x, double s, int64_t n) {
+  KrylovWork K;
+  int rc = vec_work(ctx, dtype, n, K);
+  if (!rc && n > 0) rc = K.scale(y, x, s);
+  K.release();
+  return rc;
+}
+
+extern "C" int edk_vec_axpy(edk_ctx *ctx, int dtype, void *y, const void *x, double are, double aim, int64_t n) {
+  KrylovWork K;
+  int rc = vec_work(ctx, dtype, n, K);
+  if (!rc && n > 0) rc = K.axpy(y, x, make_double2(are, aim));
+  K.release();
+  return rc;
+}
+
+extern "C" int edk_vec_dot(edk_ctx *ctx, int dtype, const void *x, const void *y, int64_t n, double *re, double *im) {
+  if (!re || !im) return edk_fail(EDK_ERR_ARG, "null output");
+  KrylovWork K;
+  int rc = vec_work(ctx, dtype, n, K);
+  double2 d = make_double2(0, 0);
+  if (!rc && n > 0) rc = K.dot(x, y, &d);
+  K.release();
+  *re = d.x;
+  *im = d.y;
+  return rc;
+}
+
+extern "C" int edk_vec_nrm2(edk_ctx *ctx, int dtype, const void *x, int64_t n, double *out) {
+  if (!out) return edk_fail(EDK_ERR_ARG, "null output");
+  KrylovWork K;
+  int rc = vec_work(ctx, dtype, n, K);
+  double v = 0;
+  if (!rc && n > 0) rc = K.norm(x, &v);
+  K.release();
+  *out = v;
+  return rc;
+}

@@ -23,6 +23,7 @@ SYMBOLS = [
     "edk_basis_download", "edk_basis_index", "edk_basis_destroy",
     "edk_operator_create", "edk_operator_destroy", "edk_operator_count_offdiag",
     "edk_matvec", "edk_matvec_host",
+    "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2",
     "edk_lanczos_ground",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
     "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
@@ -97,6 +98,10 @@ def lib():
         "edk_operator_count_offdiag": (I32, [P, C.POINTER(C.c_int64)]),
         "edk_matvec": (I32, [P, I32, P, P, I64, I32]),
         "edk_matvec_host": (I32, [P, I32, P, P, I64, I32]),
+        "edk_vec_scale": (I32, [P, I32, P, P, D, I64]),
+        "edk_vec_axpy": (I32, [P, I32, P, P, D, D, I64]),
+        "edk_vec_dot": (I32, [P, I32, P, P, I64, C.POINTER(D), C.POINTER(D)]),
+        "edk_vec_nrm2": (I32, [P, I32, P, I64, C.POINTER(D)]),
         "edk_lanczos_ground": (I32, [P, I32, P, I32, D, I32, C.POINTER(D), C.POINTER(D),
                                      C.POINTER(C.c_int), P]),
         "edk_krylov_default_params": (None, [C.POINTER(KrylovParams)]),

@@ -150,6 +150,18 @@ int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
 int edk_matvec_host(edk_op *op, int dtype, void *y_host, const void *x_host,
                     int64_t nb, int accumulate);
 
+/* ---- device vector kernels ------------------------------------------------- */
+/* The BLAS-1 steps the Krylov loops run inline in the reference
+ * (TimeEvolution.jl:261-303: w -= beta v, dot, w -= alpha v, norm, w / beta),
+ * on device vectors of length n (dtype EDK_C64 or EDK_C32).  Reductions are
+ * FP64 with a fixed two-stage order: results are bitwise reproducible. */
+int edk_vec_scale(edk_ctx *ctx, int dtype, void *y, const void *x, double s, int64_t n);  /* y = s x */
+int edk_vec_axpy(edk_ctx *ctx, int dtype, void *y, const void *x, double a_re, double a_im,
+                 int64_t n);                                                              /* y += a x */
+int edk_vec_dot(edk_ctx *ctx, int dtype, const void *x, const void *y, int64_t n, double *re,
+                double *im);                                                              /* x^H y */
+int edk_vec_nrm2(edk_ctx *ctx, int dtype, const void *x, int64_t n, double *out);         /* ||x|| */
+
 /* ---- Krylov --------------------------------------------------------------- */
 /* Lanczos ground state (no counterpart in the reference; built from the
  * _lanczos_build! recurrence TimeEvolution.jl:250-306 with full
