@@ -28,12 +28,23 @@
 #define FAST_MAXFAM 4
 #define FAST_MAXENT 12
 #define FAST_TPB 256
+#define FAST_TILE 4096
 
 struct FamEnt {
-  int sel;     // bit 1: first site must be 1, bit 0: second site must be 1
+  int locs;    // bit loc set: the entry acts where (first, second) site digits == loc
   int flip;    // bit 1: flip first site, bit 0: flip second site
-  double val;  // M[loc, c]
+  double val;  // M[loc, c], equal for every loc in locs
 };
+
+// positions (bits of the first site) whose local configuration is in locs
+__device__ __forceinline__ uint32_t ent_mask(int locs, uint32_t s, uint32_t ns, uint32_t r, uint32_t mask) {
+  uint32_t nr = ~r & mask, m = 0;
+  if (locs & 1) m |= ns & nr;
+  if (locs & 2) m |= ns & r;
+  if (locs & 4) m |= s & nr;
+  if (locs & 8) m |= s & r;
+  return m;
+}
 
 struct FastFam {
   int d;
@@ -63,6 +74,7 @@ struct FastArgs {
   uint32_t ndir;
   uint32_t D;
   const double *Rrow;
+  const uint16_t *perm;  // per tile of FAST_TILE rows: row order by target count (or null)
 };
 
 __device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
@@ -167,10 +179,12 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
   // registers, so the batch of targets can be interleaved without spills
   uint32_t minA = s, cur = s;
 #pragma unroll
-  for (int k = 1; k < 32; k++) {
-    cur = __builtin_amdgcn_alignbit(cur, cur, 1);
-    minA = min(minA, cur);
+  for (int k = 0; k < 15; k++) {  // rotations 1..30 in pairs (v_min3)
+    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
+    minA = min(minA, min(c1, c2));
+    cur = c2;
   }
+  minA = min(minA, __builtin_amdgcn_alignbit(cur, cur, 1));
   int period = 32;
 #pragma unroll
   for (int k = 16; k >= 1; k >>= 1) period = (__builtin_amdgcn_alignbit(s, s, k) == s) ? k : period;
@@ -201,10 +215,12 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
   uint32_t minB = rb;
   cur = rb;
 #pragma unroll
-  for (int k = 1; k < 32; k++) {
-    cur = __builtin_amdgcn_alignbit(cur, cur, 1);
-    minB = min(minB, cur);
+  for (int k = 0; k < 15; k++) {
+    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
+    minB = min(minB, min(c1, c2));
+    cur = c2;
   }
+  minB = min(minB, __builtin_amdgcn_alignbit(cur, cur, 1));
   bool qp = (minA == minB);
   double R = A.Rtab[(qp ? 32 : 0) + period - 1];
   double sgn = 1.0;
@@ -269,101 +285,90 @@ __device__ __forceinline__ int64_t fast_find(const FastArgs &A, uint32_t key, ui
 template <int KIND, bool CPLX, int NB, int LT, typename VT, typename ST>
 __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 5 : 6))
     k_matvec_fast(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y, int accumulate) {
+  // orbit norms and phases staged in LDS (read once per target)
+  __shared__ double sR[2 * 64];
+  __shared__ double2 sP[CPLX ? 64 : 1];
+  if (KIND != EDK_PROJECTED) {
+    int nr = (KIND == EDK_TRANSLATION_PARITY ? 2 : 1) * A.len;
+    if ((int)threadIdx.x < nr) sR[threadIdx.x] = A.Rtab[threadIdx.x];
+    if (CPLX && (int)threadIdx.x < A.len) sP[threadIdx.x] = A.ph[threadIdx.x];
+    __syncthreads();
+  }
+  FastArgs B = A;
+  B.Rtab = sR;
+  if (CPLX) B.ph = sP;
   uint32_t i = blockIdx.x * FAST_TPB + threadIdx.x;
   if (i >= A.D) return;
+  // rows of a tile visited in order of their target count (optional schedule)
+  if (A.perm) i = (i & ~(FAST_TILE - 1)) + A.perm[i];
   const FastTab &T = *A.tab;
   const uint32_t s = A.keys[i];
   const uint32_t ns = ~s & A.mask;
   ST dg = 0;
   ST orr = 0, oi = 0;
-  // diagonal by popcounts
-  for (int ff = 0; ff < T.nfam; ff++) {
-    const FastFam &F = T.fam[ff];
-    uint32_t rr = rotr32(s, A.L - F.d, A.L, A.mask);  // rotl by d
-    uint32_t nr = ~rr & A.mask;
-    dg += (ST)(F.diag[0] * __popc(ns & nr) + F.diag[1] * __popc(ns & rr) + F.diag[2] * __popc(s & nr) +
-               F.diag[3] * __popc(s & rr));
-  }
-  // target generator: family f, entry e, remaining bit positions
-  int f = 0, e = -1;
-  uint32_t bits = 0;
-  uint32_t r = T.nfam > 0 ? rotr32(s, A.L - T.fam[0].d, A.L, A.mask) : 0;
-  bool more = T.nfam > 0;
-  while (true) {
-    uint32_t tg[NB];
-    double tv[NB];
-    int n = 0;
-    while (n < NB && more) {
-      if (bits == 0) {
-        e++;
-        while (f < T.nfam && e >= T.fam[f].nent) {
-          f++;
-          e = 0;
-          if (f < T.nfam) r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
-        }
-        if (f >= T.nfam) {
-          more = false;
-          break;
-        }
-        const FamEnt &E = T.fam[f].ent[e];
-        uint32_t a = (E.sel & 2) ? s : ns;
-        uint32_t b = (E.sel & 1) ? r : (~r & A.mask);
-        bits = a & b;
-        continue;
-      }
-      int q = __ffs(bits) - 1;
-      bits &= bits - 1;
-      const FamEnt &E = T.fam[f].ent[e];
-      int q2 = q - T.fam[f].d;
-      if (q2 < 0) q2 += A.L;
-      uint32_t t = s;
-      if (E.flip & 2) t ^= 1u << q;
-      if (E.flip & 1) t ^= 1u << q2;
-      tg[n] = t;
-      tv[n] = E.val;
-      n++;
-    }
-    if (n == 0) break;
-    uint32_t rep[NB], lo[NB], hi[NB];
-    ST cfr[NB], cfi[CPLX ? NB : 1];
+  // families and entries are walked by uniform loops (scalar table reads);
+  // only the bit loop over one entry's positions diverges between lanes
+  for (int f = 0; f < T.nfam; f++) {
+    const int d = T.fam[f].d;
+    const uint32_t r = rotr32(s, A.L - d, A.L, A.mask);  // rotl by d
+    const uint32_t nr = ~r & A.mask;
+    dg += (ST)(T.fam[f].diag[0] * __popc(ns & nr) + T.fam[f].diag[1] * __popc(ns & r) +
+               T.fam[f].diag[2] * __popc(s & nr) + T.fam[f].diag[3] * __popc(s & r));
+    for (int e = 0; e < T.fam[f].nent; e++) {
+      const int flip = T.fam[f].ent[e].flip;
+      const double val = T.fam[f].ent[e].val;
+      uint32_t bits = ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask);
+      while (bits) {
+        uint32_t rep[NB], lo[NB], hi[NB];
+        bool ok[NB];
+        ST cfr[NB], cfi[CPLX ? NB : 1];
 #pragma unroll
-    for (int q = 0; q < NB; q++) {
-      lo[q] = 0;
-      hi[q] = 0;
-      rep[q] = 0;
-      double cr = 0.0, ci = 0.0;
-      bool okc = false;
-      if (q < n) {
-        if (LT == 32) okc = fast_canon32<KIND, CPLX>(A, tg[q], rep[q], cr, ci);
-        else okc = fast_canon<KIND, CPLX>(A, tg[q], rep[q], cr, ci);
-      }
-      if (okc) {
-        uint32_t bkt = rep[q] >> A.dshift;
-        if (bkt < A.ndir) {
-          lo[q] = A.dir[bkt];
-          hi[q] = A.dir[bkt + 1];
+        for (int q = 0; q < NB; q++) {
+          ok[q] = bits != 0;
+          int p = __ffs(bits) - 1;
+          bits &= bits - 1;
+          int p2 = p - d;
+          if (LT == 32) p2 &= 31;
+          else if (p2 < 0) p2 += A.L;
+          uint32_t t = s;
+          if (flip & 2) t ^= 1u << p;
+          if (flip & 1) t ^= 1u << p2;
+          double cr = 0.0, ci = 0.0;
+          rep[q] = 0;
+          if (ok[q]) {
+            if (LT == 32) ok[q] = fast_canon32<KIND, CPLX>(B, t, rep[q], cr, ci);
+            else ok[q] = fast_canon<KIND, CPLX>(B, t, rep[q], cr, ci);
+          }
+          lo[q] = 0;
+          hi[q] = 0;
+          if (ok[q]) {
+            uint32_t bkt = rep[q] >> A.dshift;
+            if (bkt < A.ndir) {
+              lo[q] = A.dir[bkt];
+              hi[q] = A.dir[bkt + 1];
+            }
+          }
+          cfr[q] = (ST)(val * cr);
+          if (CPLX) cfi[q] = (ST)(val * ci);
         }
-      }
-      cfr[q] = (ST)(tv[q] * cr);
-      if (CPLX) cfi[q] = (ST)(tv[q] * ci);
-    }
-    uint32_t jj[NB];
+        uint32_t jj[NB];
 #pragma unroll
-    for (int q = 0; q < NB; q++) jj[q] = (uint32_t)fast_find(A, rep[q], lo[q], hi[q]);
+        for (int q = 0; q < NB; q++) jj[q] = (uint32_t)fast_find(A, rep[q], lo[q], hi[q]);
 #pragma unroll
-    for (int q = 0; q < NB; q++) {
-      if (jj[q] != 0xffffffffu) {
-        VT xj = x[jj[q]];
-        if (CPLX) {
-          orr += cfr[q] * (ST)xj.x - cfi[q] * (ST)xj.y;
-          oi += cfr[q] * (ST)xj.y + cfi[q] * (ST)xj.x;
-        } else {
-          orr += cfr[q] * (ST)xj.x;
-          oi += cfr[q] * (ST)xj.y;
+        for (int q = 0; q < NB; q++) {
+          if (jj[q] != 0xffffffffu) {
+            VT xj = x[jj[q]];
+            if (CPLX) {
+              orr += cfr[q] * (ST)xj.x - cfi[q] * (ST)xj.y;
+              oi += cfr[q] * (ST)xj.y + cfi[q] * (ST)xj.x;
+            } else {
+              orr += cfr[q] * (ST)xj.x;
+              oi += cfr[q] * (ST)xj.y;
+            }
+          }
         }
       }
     }
-    if (!more) break;
   }
   VT xi = x[i];
   ST invR = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[i]);
@@ -391,8 +396,23 @@ struct FastState {
   FastTab *d_tab = nullptr;
   uint32_t *d_keys32 = nullptr;
   uint32_t *d_dir = nullptr;
+  uint16_t *d_perm = nullptr;
   bool cplx = false;
 };
+
+// number of off-diagonal targets the generator produces for each row
+__global__ void k_row_targets(FastArgs A, uint8_t *cnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.D) return;
+  const FastTab &T = *A.tab;
+  uint32_t s = A.keys[i], ns = ~s & A.mask;
+  int n = 0;
+  for (int f = 0; f < T.nfam; f++) {
+    uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+    for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
+  }
+  cnt[i] = (uint8_t)min(n, 255);
+}
 
 __global__ void k_keys32(const uint64_t *k64, int64_t D, uint32_t *k32) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,6 +447,7 @@ void edk_fast_release(edk_op *op) {
   (void)hipFree(F->d_tab);
   (void)hipFree(F->d_keys32);
   (void)hipFree(F->d_dir);
+  (void)hipFree(F->d_perm);
   delete F;
   op->fast = nullptr;
 }
@@ -480,8 +501,16 @@ static bool find_families(const edk_op *op, FastTab &tab) {
           F.diag[loc] += v.x;
           continue;
         }
+        // entries that flip the same sites with the same value share one mask
+        int hit = -1;
+        for (int q2 = 0; q2 < F.nent; q2++)
+          if (F.ent[q2].flip == (loc ^ c) && F.ent[q2].val == v.x) hit = q2;
+        if (hit >= 0) {
+          F.ent[hit].locs |= 1 << loc;
+          continue;
+        }
         if (F.nent >= FAST_MAXENT) return false;
-        F.ent[F.nent].sel = loc;
+        F.ent[F.nent].locs = 1 << loc;
         F.ent[F.nent].flip = loc ^ c;
         F.ent[F.nent].val = v.x;
         F.nent++;
@@ -544,6 +573,33 @@ static bool fast_setup(edk_op *op) {
   A.ndir = ndir;
   A.D = (uint32_t)b->D;
   A.Rrow = b->d_R;
+  A.perm = nullptr;
+  // row order within tiles by target count (stable counting sort on the host)
+  const char *ps = getenv("EDK_FAST_SORT_ROWS");
+  if (ps && ps[0] == '1') {
+    uint8_t *d_cnt = nullptr;
+    std::vector<uint8_t> cnt(b->D);
+    std::vector<uint16_t> perm(b->D);
+    if (hipMalloc((void **)&d_cnt, b->D) != hipSuccess) return false;
+    k_row_targets<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
+    bool ok = hipMemcpyAsync(cnt.data(), d_cnt, b->D, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+              hipStreamSynchronize(ctx->stream) == hipSuccess;
+    (void)hipFree(d_cnt);
+    if (!ok) return false;
+    for (int64_t t0 = 0; t0 < b->D; t0 += FAST_TILE) {
+      int64_t t1 = std::min<int64_t>(b->D, t0 + FAST_TILE);
+      int hist[257] = {0};
+      for (int64_t i = t0; i < t1; i++) hist[cnt[i] + 1]++;
+      for (int c = 0; c < 256; c++) hist[c + 1] += hist[c];
+      for (int64_t i = t0; i < t1; i++) perm[t0 + hist[cnt[i]]++] = (uint16_t)(i - t0);
+    }
+    if (hipMalloc((void **)&S->d_perm, b->D * sizeof(uint16_t)) != hipSuccess) return false;
+    if (hipMemcpyAsync(S->d_perm, perm.data(), b->D * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return false;
+    A.perm = S->d_perm;
+  }
   S->ok = true;
   return true;
 }
