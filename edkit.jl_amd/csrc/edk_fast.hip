@@ -91,7 +91,7 @@ __device__ __forceinline__ uint32_t rev32(uint32_t s, int L) { return __brev(s) 
 // divergence): the minimum over a whole cycle of len rotations equals the
 // minimum over one period, and the first index attaining it lies in the
 // first period, as in the reference's loops that stop at the period.
-template <int KIND, bool CPLX>
+template <int KIND, int PH>
 __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
                                            double &ci) {
   if (KIND == EDK_PROJECTED) {
@@ -116,7 +116,7 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
     rep = minA;
     if (((int64_t)period * A.kp) % A.len) return false;  // check_momentum
     double R = A.Rtab[period - 1];
-    if (CPLX || A.kp != 0) {
+    if (PH != 0) {
       int M = 0;
       cur = s;
       for (int i = 1; i < A.len; i++) {
@@ -144,7 +144,7 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
   bool qp = (minA == minB);
   double R = A.Rtab[(qp ? A.len : 0) + period - 1];
   double sgn = 1.0;
-  if (A.kp != 1 || A.P != 1) {
+  if (PH != 0) {
     // first index M with rot^M(s) (or reverse(rot^M(s))) at the minimum
     // (TranslationalParityBasis.jl:207-231)
     int Ma = -1, Mb = -1;
@@ -166,7 +166,7 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
 // The same canonicaliser for L = 32 and unit cell 1, fully unrolled: each of
 // the 31 rotations is one v_alignbit, the minima fold into v_min3, the
 // orbit period (a power of two dividing 32) is read off the same rotations.
-template <int KIND, bool CPLX>
+template <int KIND, int PH>
 __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
                                              double &ci) {
   if (KIND == EDK_PROJECTED) {
@@ -192,7 +192,7 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
     rep = minA;
     if (((int64_t)period * A.kp) % 32) return false;  // check_momentum
     double R = A.Rtab[period - 1];
-    if (CPLX || A.kp != 0) {
+    if (PH != 0) {
       int M = 0;
       bool found = (s == minA);
       cur = s;
@@ -224,7 +224,7 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
   bool qp = (minA == minB);
   double R = A.Rtab[(qp ? 32 : 0) + period - 1];
   double sgn = 1.0;
-  if (A.kp != 1 || A.P != 1) {
+  if (PH != 0) {
     // first k with rot^k(s) == minA, resp. reverse(rot^k(s)) == minB
     // (TranslationalParityBasis.jl:207-231)
     int Ma = 0, Mb = 0;
@@ -282,21 +282,22 @@ __device__ __forceinline__ int64_t fast_find(const FastArgs &A, uint32_t key, ui
   return (lo < A.D && A.keys[lo] == key) ? (int64_t)lo : -1;
 }
 
-template <int KIND, bool CPLX, int NB, int LT, typename VT, typename ST>
-__global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 5 : 6))
+template <int KIND, int PH, int NB, int LT, typename VT, typename ST>
+__global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
     k_matvec_fast(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y, int accumulate) {
+  constexpr bool CPLX = (PH == 2);
   // orbit norms and phases staged in LDS (read once per target)
   __shared__ double sR[2 * 64];
-  __shared__ double2 sP[CPLX ? 64 : 1];
+  __shared__ double2 sP[PH != 0 ? 64 : 1];
   if (KIND != EDK_PROJECTED) {
     int nr = (KIND == EDK_TRANSLATION_PARITY ? 2 : 1) * A.len;
     if ((int)threadIdx.x < nr) sR[threadIdx.x] = A.Rtab[threadIdx.x];
-    if (CPLX && (int)threadIdx.x < A.len) sP[threadIdx.x] = A.ph[threadIdx.x];
+    if (PH != 0 && KIND == EDK_TRANSLATIONAL && (int)threadIdx.x < A.len) sP[threadIdx.x] = A.ph[threadIdx.x];
     __syncthreads();
   }
   FastArgs B = A;
   B.Rtab = sR;
-  if (CPLX) B.ph = sP;
+  if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
   uint32_t i = blockIdx.x * FAST_TPB + threadIdx.x;
   if (i >= A.D) return;
   // rows of a tile visited in order of their target count (optional schedule)
@@ -336,8 +337,8 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 5 : 6))
           double cr = 0.0, ci = 0.0;
           rep[q] = 0;
           if (ok[q]) {
-            if (LT == 32) ok[q] = fast_canon32<KIND, CPLX>(B, t, rep[q], cr, ci);
-            else ok[q] = fast_canon<KIND, CPLX>(B, t, rep[q], cr, ci);
+            if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
+            else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
           }
           lo[q] = 0;
           hi[q] = 0;
@@ -604,27 +605,27 @@ static bool fast_setup(edk_op *op) {
   return true;
 }
 
-template <int KIND, bool CPLX, int NB, int LT>
+template <int KIND, int PH, int NB, int LT>
 static void launch_fast_nb(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
   unsigned grid = (unsigned)((A.D + FAST_TPB - 1) / FAST_TPB);
   if (dtype == EDK_C64)
-    k_matvec_fast<KIND, CPLX, NB, LT, double2, double>
+    k_matvec_fast<KIND, PH, NB, LT, double2, double>
         <<<grid, FAST_TPB, 0, st>>>(A, (const double2 *)x, (double2 *)y, acc);
   else
-    k_matvec_fast<KIND, CPLX, NB, LT, float2, float><<<grid, FAST_TPB, 0, st>>>(A, (const float2 *)x, (float2 *)y, acc);
+    k_matvec_fast<KIND, PH, NB, LT, float2, float><<<grid, FAST_TPB, 0, st>>>(A, (const float2 *)x, (float2 *)y, acc);
 }
 
-template <int KIND, bool CPLX, int LT>
+template <int KIND, int PH, int LT>
 static void launch_fast_lt(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
-  if (A.batch >= 8) launch_fast_nb<KIND, CPLX, 8, LT>(A, dtype, y, x, acc, st);
-  else if (A.batch >= 4) launch_fast_nb<KIND, CPLX, 4, LT>(A, dtype, y, x, acc, st);
-  else launch_fast_nb<KIND, CPLX, 2, LT>(A, dtype, y, x, acc, st);
+  if (A.batch >= 8) launch_fast_nb<KIND, PH, 8, LT>(A, dtype, y, x, acc, st);
+  else if (A.batch >= 4) launch_fast_nb<KIND, PH, 4, LT>(A, dtype, y, x, acc, st);
+  else launch_fast_nb<KIND, PH, 2, LT>(A, dtype, y, x, acc, st);
 }
 
-template <int KIND, bool CPLX>
+template <int KIND, int PH>
 static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
-  if (A.L == 32 && A.A == 1) launch_fast_lt<KIND, CPLX, 32>(A, dtype, y, x, acc, st);
-  else launch_fast_lt<KIND, CPLX, 0>(A, dtype, y, x, acc, st);
+  if (A.L == 32 && A.A == 1) launch_fast_lt<KIND, PH, 32>(A, dtype, y, x, acc, st);
+  else launch_fast_lt<KIND, PH, 0>(A, dtype, y, x, acc, st);
 }
 
 bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
@@ -633,13 +634,18 @@ bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, 
   FastState *S = (FastState *)op->fast;
   const FastArgs &A = S->args;
   hipStream_t st = op->ctx->stream;
+  // phase mode: 0 all coefficients real and positive, 1 real signs, 2 complex
   switch (A.kind) {
-    case EDK_TRANSLATION_PARITY: launch_fast<EDK_TRANSLATION_PARITY, false>(A, dtype, y, x, accumulate, st); break;
-    case EDK_TRANSLATIONAL:
-      if (S->cplx) launch_fast<EDK_TRANSLATIONAL, true>(A, dtype, y, x, accumulate, st);
-      else launch_fast<EDK_TRANSLATIONAL, false>(A, dtype, y, x, accumulate, st);
+    case EDK_TRANSLATION_PARITY:
+      if (A.kp == 1 && A.P == 1) launch_fast<EDK_TRANSLATION_PARITY, 0>(A, dtype, y, x, accumulate, st);
+      else launch_fast<EDK_TRANSLATION_PARITY, 1>(A, dtype, y, x, accumulate, st);
       break;
-    case EDK_PROJECTED: launch_fast<EDK_PROJECTED, false>(A, dtype, y, x, accumulate, st); break;
+    case EDK_TRANSLATIONAL:
+      if (A.kp == 0) launch_fast<EDK_TRANSLATIONAL, 0>(A, dtype, y, x, accumulate, st);
+      else if (2 * A.kp == A.len) launch_fast<EDK_TRANSLATIONAL, 1>(A, dtype, y, x, accumulate, st);
+      else launch_fast<EDK_TRANSLATIONAL, 2>(A, dtype, y, x, accumulate, st);
+      break;
+    case EDK_PROJECTED: launch_fast<EDK_PROJECTED, 0>(A, dtype, y, x, accumulate, st); break;
     default: return false;
   }
   return hipGetLastError() == hipSuccess;
