@@ -20,7 +20,9 @@
 // together, so each lane keeps eight independent memory requests in flight.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "edk_matvec.h"
@@ -75,6 +77,14 @@ struct FastArgs {
   uint32_t D;
   const double *Rrow;
   const uint16_t *perm;  // per tile of FAST_TILE rows: row order by target count (or null)
+  const uint2 *region;   // two-level directory: per region (key >> rshift) {base index, fine offset << 5 | shift}
+  int rshift;            // region width in key bits
+  const uint16_t *fine;  // per region: positions relative to base, one slot per 2^shift key values
+  const uint32_t *chunk; // balanced schedule: first row of each chunk (nchunk + 1 entries), or null
+  uint32_t nchunk;
+  const uint4 *htab;     // hash index: buckets of 8 (key, position) slots, or null
+  uint32_t hbuckets;     // number of buckets
+  int hprobe;            // longest probe sequence in buckets (>= 1)
 };
 
 __device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
@@ -248,6 +258,27 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
   return true;
 }
 
+#define HASH_EMPTY 0xffffffffu
+__device__ __forceinline__ uint32_t hash_bucket(uint32_t key, uint32_t nb) {
+  return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
+}
+
+// One bucket of the hash index: 8 (key, position) slots in 64 bytes.
+// Returns the position, HASH_EMPTY when the bucket proves the key absent
+// (it has a free slot), or HASH_EMPTY - 1 when the next bucket must be read.
+__device__ __forceinline__ uint32_t hash_probe(const uint4 *b, uint32_t key) {
+  uint4 w0 = b[0], w1 = b[1], w2 = b[2], w3 = b[3];
+  uint32_t r = HASH_EMPTY - 1;
+  bool empty = false;
+#define HSLOT(K, V)                         \
+  r = ((K) == key && (V) != HASH_EMPTY) ? (V) : r; \
+  empty = empty || (V) == HASH_EMPTY;
+  HSLOT(w0.x, w0.y) HSLOT(w0.z, w0.w) HSLOT(w1.x, w1.y) HSLOT(w1.z, w1.w)
+  HSLOT(w2.x, w2.y) HSLOT(w2.z, w2.w) HSLOT(w3.x, w3.y) HSLOT(w3.z, w3.w)
+#undef HSLOT
+  return (r == HASH_EMPTY - 1 && empty) ? HASH_EMPTY : r;
+}
+
 // Position of key in the sorted table, or -1.  The directory bounds the
 // search to [lo, hi); a bucket of at most A.scan keys is searched with one
 // set of independent 16-byte loads (one memory round), a larger one by binary
@@ -280,6 +311,43 @@ __device__ __forceinline__ int64_t fast_find(const FastArgs &A, uint32_t key, ui
     else hi = mid;
   }
   return (lo < A.D && A.keys[lo] == key) ? (int64_t)lo : -1;
+}
+
+// Two-level directory: region r = key >> rshift gives the region's first table
+// index and its fine directory (sized to the region's population); the fine
+// bucket bounds the key to [lo, hi), usually at most four keys, which are read
+// with independent loads in one round.
+#define FINE_SCAN 4
+__device__ __forceinline__ void region_bounds(const FastArgs &A, uint32_t key, uint32_t &lo, uint32_t &hi) {
+  uint2 rg = A.region[key >> A.rshift];
+  uint32_t sh = rg.y & 31, foff = rg.y >> 5;
+  uint32_t fb = (key & ((1u << A.rshift) - 1)) >> sh;
+  lo = rg.x + A.fine[foff + fb];
+  hi = rg.x + A.fine[foff + fb + 1];
+}
+
+__device__ __forceinline__ uint32_t region_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
+  if (hi <= lo) return HASH_EMPTY;
+  if (hi - lo <= FINE_SCAN) {
+    uint32_t below = 0;
+    bool eq = false;
+#pragma unroll
+    for (int k = 0; k < FINE_SCAN; k++) {
+      uint32_t v = A.keys[lo + k];  // padded table: reads past the bucket are harmless
+      bool in = lo + k < hi;
+      below += (in && v < key);
+      eq = eq || (in && v == key);
+    }
+    return eq ? lo + below : HASH_EMPTY;
+  }
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint32_t v = A.keys[mid];
+    if (v == key) return mid;
+    if (v < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return HASH_EMPTY;
 }
 
 template <int KIND, int PH, int NB, int LT, typename VT, typename ST>
@@ -342,7 +410,9 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
           }
           lo[q] = 0;
           hi[q] = 0;
-          if (ok[q]) {
+          if (ok[q] && A.region) {
+            region_bounds(A, rep[q], lo[q], hi[q]);
+          } else if (ok[q] && !A.htab) {
             uint32_t bkt = rep[q] >> A.dshift;
             if (bkt < A.ndir) {
               lo[q] = A.dir[bkt];
@@ -353,8 +423,31 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
           if (CPLX) cfi[q] = (ST)(val * ci);
         }
         uint32_t jj[NB];
+        if (A.region) {
 #pragma unroll
-        for (int q = 0; q < NB; q++) jj[q] = (uint32_t)fast_find(A, rep[q], lo[q], hi[q]);
+          for (int q = 0; q < NB; q++) jj[q] = region_find(A, rep[q], lo[q], hi[q]);
+        } else if (A.htab) {
+          // home buckets of all targets first (independent loads), then the
+          // rare overflow into the following buckets
+          uint32_t hb[NB];
+#pragma unroll
+          for (int q = 0; q < NB; q++) {
+            hb[q] = hash_bucket(rep[q], A.hbuckets);
+            jj[q] = ok[q] ? hash_probe(A.htab + 4 * (size_t)hb[q], rep[q]) : HASH_EMPTY;
+          }
+#pragma unroll
+          for (int q = 0; q < NB; q++) {
+            for (int t = 1; t < A.hprobe && jj[q] == HASH_EMPTY - 1; t++) {
+              uint32_t b2 = hb[q] + t;
+              if (b2 >= A.hbuckets) b2 -= A.hbuckets;
+              jj[q] = hash_probe(A.htab + 4 * (size_t)b2, rep[q]);
+            }
+            if (jj[q] == HASH_EMPTY - 1) jj[q] = HASH_EMPTY;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < NB; q++) jj[q] = (uint32_t)fast_find(A, rep[q], lo[q], hi[q]);
+        }
 #pragma unroll
         for (int q = 0; q < NB; q++) {
           if (jj[q] != 0xffffffffu) {
@@ -388,6 +481,177 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
 }
 
 // ---------------------------------------------------------------------------
+// balanced kernel: one block per chunk of whole rows holding at most
+// BAL_CAP targets.  Phase 1: the row threads write their targets (and entry
+// numbers) into LDS in generator order.  Phase 2: every thread takes BAL_K
+// targets (j = tid + BAL_TPB q), canonicalises them, looks them up and gathers
+// x, all BAL_K independent so they overlap.  Phase 3: each row thread sums its
+// contiguous slice of contributions in order and writes y.
+// ---------------------------------------------------------------------------
+#define BAL_TPB 256
+#define BAL_K 4
+#define BAL_CAP (BAL_TPB * BAL_K)
+
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total, uint32_t *wsum) {
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t xx = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t yv = __shfl_up(xx, o, 64);
+    if (lane >= o) xx += yv;
+  }
+  if (lane == 63) wsum[wid] = xx;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int q = 0; q < BAL_TPB / 64; q++) {
+    if (q < wid) base += wsum[q];
+    tot += wsum[q];
+  }
+  *total = tot;
+  return base + xx - v;
+}
+
+template <int KIND, int PH, int LT, typename VT, typename ST>
+__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
+                                                         int accumulate) {
+  constexpr bool CPLX = (PH == 2);
+  __shared__ double sR[2 * 64];
+  __shared__ double2 sP[PH != 0 ? 64 : 1];
+  __shared__ double sV[FAST_MAXFAM * FAST_MAXENT];
+  __shared__ uint32_t tgt[BAL_CAP];
+  __shared__ uint8_t ent[BAL_CAP];
+  __shared__ ST contr[CPLX ? 2 : 2][BAL_CAP];
+  __shared__ uint32_t wsum[BAL_TPB / 64];
+  const FastTab &T = *A.tab;
+  const int tid = threadIdx.x;
+  if (KIND != EDK_PROJECTED) {
+    int nr = (KIND == EDK_TRANSLATION_PARITY ? 2 : 1) * A.len;
+    if (tid < nr) sR[tid] = A.Rtab[tid];
+    if (PH != 0 && KIND == EDK_TRANSLATIONAL && tid < A.len) sP[tid] = A.ph[tid];
+  }
+  if (tid < FAST_MAXFAM * FAST_MAXENT) {
+    int f = tid / FAST_MAXENT, e = tid % FAST_MAXENT;
+    sV[tid] = (f < T.nfam && e < T.fam[f].nent) ? T.fam[f].ent[e].val : 0.0;
+  }
+  FastArgs B = A;
+  B.Rtab = sR;
+  if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
+  const uint32_t r0 = A.chunk[blockIdx.x], nrow = A.chunk[blockIdx.x + 1] - r0;
+  // phase 1: row threads count and write their targets
+  uint32_t s = 0, n = 0;
+  const bool own = (uint32_t)tid < nrow;
+  if (own) {
+    s = A.keys[r0 + tid];
+    const uint32_t ns = ~s & A.mask;
+    for (int f = 0; f < T.nfam; f++) {
+      uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
+    }
+  }
+  uint32_t total;
+  const uint32_t off = block_scan_u32(n, &total, wsum);
+  if (own) {
+    const uint32_t ns = ~s & A.mask;
+    uint32_t o = off;
+    for (int f = 0; f < T.nfam; f++) {
+      const int d = T.fam[f].d;
+      uint32_t r = rotr32(s, A.L - d, A.L, A.mask);
+      for (int e = 0; e < T.fam[f].nent; e++) {
+        const int flip = T.fam[f].ent[e].flip;
+        uint32_t bits = ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask);
+        while (bits) {
+          int p = __ffs(bits) - 1;
+          bits &= bits - 1;
+          int p2 = p - d;
+          if (LT == 32) p2 &= 31;
+          else if (p2 < 0) p2 += A.L;
+          uint32_t t = s;
+          if (flip & 2) t ^= 1u << p;
+          if (flip & 1) t ^= 1u << p2;
+          tgt[o] = t;
+          ent[o] = (uint8_t)(f * FAST_MAXENT + e);
+          o++;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2: BAL_K targets per thread, no divergence
+  uint32_t rep[BAL_K], lo[BAL_K], hi[BAL_K], jj[BAL_K];
+  bool ok[BAL_K];
+  ST cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
+#pragma unroll
+  for (int q = 0; q < BAL_K; q++) {
+    const uint32_t j = tid + BAL_TPB * q;
+    ok[q] = j < total;
+    uint32_t t = ok[q] ? tgt[j] : 0u;
+    double cr = 0.0, ci = 0.0;
+    rep[q] = 0;
+    if (ok[q]) {
+      if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
+      else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
+    }
+    lo[q] = hi[q] = 0;
+    if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
+    double v = ok[q] ? sV[ent[j]] : 0.0;
+    cfr[q] = (ST)(v * cr);
+    if (CPLX) cfi[q] = (ST)(v * ci);
+  }
+#pragma unroll
+  for (int q = 0; q < BAL_K; q++) jj[q] = ok[q] ? region_find(A, rep[q], lo[q], hi[q]) : HASH_EMPTY;
+#pragma unroll
+  for (int q = 0; q < BAL_K; q++) {
+    const uint32_t j = tid + BAL_TPB * q;
+    ST cr_ = 0, ci_ = 0;
+    if (jj[q] != HASH_EMPTY) {
+      VT xj = x[jj[q]];
+      if (CPLX) {
+        cr_ = cfr[q] * (ST)xj.x - cfi[q] * (ST)xj.y;
+        ci_ = cfr[q] * (ST)xj.y + cfi[q] * (ST)xj.x;
+      } else {
+        cr_ = cfr[q] * (ST)xj.x;
+        ci_ = cfr[q] * (ST)xj.y;
+      }
+    }
+    if (j < total) {
+      contr[0][j] = cr_;
+      contr[1][j] = ci_;
+    }
+  }
+  __syncthreads();
+  // phase 3: row sums in generator order, diagonal, output
+  if (own) {
+    const uint32_t i = r0 + tid;
+    const uint32_t ns = ~s & A.mask;
+    ST dg = 0;
+    for (int f = 0; f < T.nfam; f++) {
+      uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      uint32_t nr = ~r & A.mask;
+      dg += (ST)(T.fam[f].diag[0] * __popc(ns & nr) + T.fam[f].diag[1] * __popc(ns & r) +
+                 T.fam[f].diag[2] * __popc(s & nr) + T.fam[f].diag[3] * __popc(s & r));
+    }
+    ST orr = 0, oi = 0;
+    for (uint32_t q = off; q < off + n; q++) {
+      orr += contr[0][q];
+      oi += contr[1][q];
+    }
+    VT xi = x[i];
+    ST invR = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[i]);
+    ST yr = dg * (ST)xi.x + orr * invR;
+    ST yi = dg * (ST)xi.y + oi * invR;
+    VT out;
+    if (accumulate) {
+      VT y0 = y[i];
+      out.x = y0.x + yr;
+      out.y = y0.y + yi;
+    } else {
+      out.x = yr;
+      out.y = yi;
+    }
+    y[i] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host: eligibility and setup (cached on the operator)
 // ---------------------------------------------------------------------------
 struct FastState {
@@ -398,6 +662,10 @@ struct FastState {
   uint32_t *d_keys32 = nullptr;
   uint32_t *d_dir = nullptr;
   uint16_t *d_perm = nullptr;
+  uint4 *d_htab = nullptr;
+  uint2 *d_region = nullptr;
+  uint16_t *d_fine = nullptr;
+  uint32_t *d_chunk = nullptr;
   bool cplx = false;
 };
 
@@ -449,6 +717,10 @@ void edk_fast_release(edk_op *op) {
   (void)hipFree(F->d_keys32);
   (void)hipFree(F->d_dir);
   (void)hipFree(F->d_perm);
+  (void)hipFree(F->d_htab);
+  (void)hipFree(F->d_region);
+  (void)hipFree(F->d_fine);
+  (void)hipFree(F->d_chunk);
   delete F;
   op->fast = nullptr;
 }
@@ -575,6 +847,105 @@ static bool fast_setup(edk_op *op) {
   A.D = (uint32_t)b->D;
   A.Rrow = b->d_R;
   A.perm = nullptr;
+  A.htab = nullptr;
+  A.hbuckets = 0;
+  A.hprobe = 1;
+  A.region = nullptr;
+  A.fine = nullptr;
+  const char *lk = getenv("EDK_FAST_LOOKUP");
+  if (!lk || strcmp(lk, "region") == 0) {
+    // two-level directory (4096 regions of 2^20 key values); fine slots
+    // ~ one per representative; entries are 16-bit offsets from the region's
+    // first index, so a region may hold at most 65535 representatives
+    std::vector<uint32_t> keys(b->D);
+    if (hipMemcpyAsync(keys.data(), S->d_keys32, b->D * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return false;
+    // regions: the fewest top bits (12..16) that keep every region under 65536 keys
+    int L = b->L, RS = 0;
+    std::vector<uint32_t> first;
+    bool fits = false;
+    for (int RB = 12; RB <= 16 && !fits; RB++) {
+      RS = std::max(L - RB, 0);
+      uint32_t nreg = 1u << std::min(RB, L);
+      first.assign(nreg + 1, 0);
+      for (int64_t i = 0, r = 0; r <= (int64_t)nreg; r++) {
+        while (i < b->D && (int64_t)(keys[i] >> RS) < r) i++;
+        first[r] = (uint32_t)i;
+      }
+      fits = true;
+      for (uint32_t r = 0; r < nreg; r++)
+        if (first[r + 1] - first[r] > 65535) fits = false;
+    }
+    if (fits) {
+      uint32_t nreg = (uint32_t)first.size() - 1;
+      std::vector<uint2> region(nreg);
+      std::vector<uint16_t> fine;
+      for (uint32_t r = 0; r < nreg; r++) {
+        uint32_t n = first[r + 1] - first[r];
+        int sh = RS;
+        while (sh > 0 && (1u << (RS - sh)) < std::max<uint32_t>(n, 1)) sh--;
+        uint32_t nf = 1u << (RS - sh);
+        region[r] = make_uint2(first[r], ((uint32_t)fine.size() << 5) | (uint32_t)sh);
+        uint32_t i = first[r];
+        for (uint32_t fb = 0; fb <= nf; fb++) {
+          uint64_t lim = ((uint64_t)r << RS) + ((uint64_t)fb << sh);
+          while (i < first[r + 1] && (uint64_t)keys[i] < lim) i++;
+          fine.push_back((uint16_t)(i - first[r]));
+        }
+      }
+      if (fine.size() >= (1u << 27)) fits = false;
+      if (fits) {
+        if (hipMalloc((void **)&S->d_region, region.size() * sizeof(uint2)) != hipSuccess ||
+            hipMalloc((void **)&S->d_fine, fine.size() * sizeof(uint16_t)) != hipSuccess)
+          return false;
+        if (hipMemcpyAsync(S->d_region, region.data(), region.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                           ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(S->d_fine, fine.data(), fine.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                           ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+          return false;
+        A.rshift = RS;
+        A.region = S->d_region;
+        A.fine = S->d_fine;
+      }
+    }
+  }
+  if (lk && strcmp(lk, "hash") == 0) {
+    // hash index over the representatives: bucketised linear probing, built
+    // on the host in table order (deterministic), ~70% slot load
+    std::vector<uint32_t> keys(b->D);
+    if (hipMemcpyAsync(keys.data(), S->d_keys32, b->D * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return false;
+    uint32_t nb = (uint32_t)std::max<int64_t>(1, (int64_t)std::ceil(b->D / (8 * 0.7)));
+    std::vector<uint32_t> tab((size_t)nb * 16, 0xffffffffu);
+    int maxp = 1;
+    for (int64_t i = 0; i < b->D; i++) {
+      uint32_t key = keys[i];
+      uint32_t bk = (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
+      for (int t = 0;; t++) {
+        uint32_t *bb = &tab[(size_t)bk * 16];
+        int slot = -1;
+        for (int q = 0; q < 8 && slot < 0; q++)
+          if (bb[2 * q + 1] == 0xffffffffu) slot = q;
+        if (slot >= 0) {
+          bb[2 * slot] = key;
+          bb[2 * slot + 1] = (uint32_t)i;
+          maxp = std::max(maxp, t + 1);
+          break;
+        }
+        bk = (bk + 1 == nb) ? 0 : bk + 1;
+      }
+    }
+    if (hipMalloc((void **)&S->d_htab, tab.size() * 4) != hipSuccess) return false;
+    if (hipMemcpyAsync(S->d_htab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return false;
+    A.htab = S->d_htab;
+    A.hbuckets = nb;
+    A.hprobe = maxp;
+  }
   // row order within tiles by target count (stable counting sort on the host)
   const char *ps = getenv("EDK_FAST_SORT_ROWS");
   if (ps && ps[0] == '1') {
@@ -601,6 +972,48 @@ static bool fast_setup(edk_op *op) {
       return false;
     A.perm = S->d_perm;
   }
+  // balanced schedule: chunks of whole rows with at most BAL_CAP targets
+  A.chunk = nullptr;
+  A.nchunk = 0;
+  const char *sch = getenv("EDK_FAST_SCHEDULE");
+  if (A.region && (!sch || strcmp(sch, "balanced") == 0)) {
+    uint8_t *d_cnt = nullptr;
+    std::vector<uint8_t> cnt(b->D);
+    if (hipMalloc((void **)&d_cnt, b->D) != hipSuccess) return false;
+    k_row_targets<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
+    bool okc = hipMemcpyAsync(cnt.data(), d_cnt, b->D, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+               hipStreamSynchronize(ctx->stream) == hipSuccess;
+    (void)hipFree(d_cnt);
+    if (!okc) return false;
+    std::vector<uint32_t> cs;
+    bool fit = true;
+    uint32_t acc = 0, rows = 0;
+    cs.push_back(0);
+    for (int64_t i = 0; i < b->D; i++) {
+      if (cnt[i] > BAL_CAP) fit = false;
+      if (rows == BAL_TPB || acc + cnt[i] > BAL_CAP) {
+        cs.push_back((uint32_t)i);
+        acc = 0;
+        rows = 0;
+      }
+      acc += cnt[i];
+      rows++;
+    }
+    cs.push_back((uint32_t)b->D);
+    if (fit) {
+      if (hipMalloc((void **)&S->d_chunk, cs.size() * sizeof(uint32_t)) != hipSuccess) return false;
+      if (hipMemcpyAsync(S->d_chunk, cs.data(), cs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) !=
+              hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return false;
+      A.chunk = S->d_chunk;
+      A.nchunk = (uint32_t)cs.size() - 1;
+    }
+  }
+  const char *vb = getenv("EDK_FAST_VERBOSE");
+  if (vb && vb[0] == '1')
+    fprintf(stderr, "[edk fast] D=%u L=%d lookup=%s schedule=%s chunks=%u batch=%d\n", A.D, A.L,
+            A.region ? "region" : (A.htab ? "hash" : "dir"), A.chunk ? "balanced" : "lane", A.nchunk, A.batch);
   S->ok = true;
   return true;
 }
@@ -622,8 +1035,21 @@ static void launch_fast_lt(const FastArgs &A, int dtype, void *y, const void *x,
   else launch_fast_nb<KIND, PH, 2, LT>(A, dtype, y, x, acc, st);
 }
 
+template <int KIND, int PH, int LT>
+static void launch_bal_lt(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
+  if (dtype == EDK_C64)
+    k_matvec_bal<KIND, PH, LT, double2, double><<<A.nchunk, BAL_TPB, 0, st>>>(A, (const double2 *)x, (double2 *)y, acc);
+  else
+    k_matvec_bal<KIND, PH, LT, float2, float><<<A.nchunk, BAL_TPB, 0, st>>>(A, (const float2 *)x, (float2 *)y, acc);
+}
+
 template <int KIND, int PH>
 static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
+  if (A.chunk) {
+    if (A.L == 32 && A.A == 1) launch_bal_lt<KIND, PH, 32>(A, dtype, y, x, acc, st);
+    else launch_bal_lt<KIND, PH, 0>(A, dtype, y, x, acc, st);
+    return;
+  }
   if (A.L == 32 && A.A == 1) launch_fast_lt<KIND, PH, 32>(A, dtype, y, x, acc, st);
   else launch_fast_lt<KIND, PH, 0>(A, dtype, y, x, acc, st);
 }
