@@ -10,7 +10,9 @@ roofline: algorithmic bytes per matvec
     B_alg = D * (s_I + s_R + 2 s_v) + NNZ_off * (s_v + s_I)
 (s_I = 8 representative, s_R = 8 norm, s_v = 16 ComplexF64; NNZ_off counted
 exactly on the device) over the kernel's mean duration, timed with HIP events
-on the stream the kernel runs on.  cpu_baseline: the oracle's threaded scatter
+on the stream the kernel runs on.  roofline.traffic: HBM bytes per launch from
+the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same kernel and
+workload (profiles/matvec_bal_L32_tp_k0.md).  cpu_baseline: the oracle's threaded scatter
 mul on a strided sample of columns of the same matvec, scaled to H*psi/s.
 """
 import argparse
@@ -29,6 +31,21 @@ for p in (ROOT, os.path.join(ROOT, "edkit.jl_amd")):
 
 HBM_PEAK_GBS = 8000.0
 L, NUP, DELTA = 32, 16, 1.0
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the default matvec kernel on this
+# workload, scaled by the copy-probe calibration (a measured figure, not live)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "matvec_bal_L32_tp_k0.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch recorded in the committed rocprofv3 summary, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch ="):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 class HipEvents:
@@ -164,7 +181,7 @@ def main():
                                "ComplexF64 H*psi", "L": L, "D": D, "nnz_offdiag": nnz_off,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "bytes_alg_per_launch": b_alg, "kernel_ms": kern_ms},
         "basis_build_s": t_enum,
     }
