@@ -25,6 +25,7 @@ struct KrylovWork {
   int combine(const void *V, int64_t ldv, int j, const double2 *c, double scale, void *out, int out_dtype);
 };
 
-int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z);
+int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z, int nr,
+                    const int *rows);
 int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta, int *m,
                    int m_target, int64_t *matvecs, int ncols);

@@ -107,20 +107,54 @@ __global__ void k_scale(VT *y, const VS *x, double s, int64_t n) {
   st(y, i, make_double2(b.x * s, b.y * s));
 }
 
-// part[c * nblk + b] = block b's share of conj(V[:, c]) . w,  c < j
+// part[c * nblk + b] = block b's share of conj(V[:, c]) . w,  c < j.
+// Columns go in groups of GEMV_CG: each thread loads w_i once per group and
+// keeps one accumulator per column, so w is read j / GEMV_CG times instead of
+// j times; the group's block sums are reduced together (wave shuffles, then
+// one LDS pass over the waves), in a fixed order.
+#define GEMV_CG 8
 template <typename VT>
 __global__ void __launch_bounds__(RED_TPB) k_gemv_t(const VT *V, int64_t ld_v, int j, const VT *w, int64_t n,
                                                     double2 *part) {
-  for (int c = 0; c < j; c++) {
-    const VT *vc = V + (int64_t)c * ld_v;
-    double2 acc = make_double2(0, 0);
+  __shared__ double2 sm[RED_TPB / 64][GEMV_CG];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < j; c0 += GEMV_CG) {
+    const int cg = min(GEMV_CG, j - c0);
+    double2 acc[GEMV_CG];
+#pragma unroll
+    for (int q = 0; q < GEMV_CG; q++) acc[q] = make_double2(0, 0);
     for (int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_TPB) {
-      double2 x = ld(vc, i), y = ld(w, i);
-      acc.x += x.x * y.x + x.y * y.y;
-      acc.y += x.x * y.y - x.y * y.x;
+      const double2 y = ld(w, i);
+#pragma unroll
+      for (int q = 0; q < GEMV_CG; q++) {
+        if (q < cg) {
+          const double2 x = ld(V + (int64_t)(c0 + q) * ld_v, i);
+          acc[q].x += x.x * y.x + x.y * y.y;
+          acc[q].y += x.x * y.y - x.y * y.x;
+        }
+      }
     }
-    double2 r = block_sum2(acc);
-    if (threadIdx.x == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = r;
+#pragma unroll
+    for (int q = 0; q < GEMV_CG; q++) {
+      for (int o = 32; o > 0; o >>= 1) {
+        acc[q].x += __shfl_down(acc[q].x, o, 64);
+        acc[q].y += __shfl_down(acc[q].y, o, 64);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < GEMV_CG; q++) sm[wid][q] = acc[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < cg) {
+      double2 r = make_double2(0, 0);
+      for (int u = 0; u < RED_TPB / 64; u++) {
+        r.x += sm[u][threadIdx.x].x;
+        r.y += sm[u][threadIdx.x].y;
+      }
+      part[(int64_t)(c0 + threadIdx.x) * gridDim.x + blockIdx.x] = r;
+    }
   }
 }
 
@@ -258,12 +292,18 @@ int KrylovWork::combine(const void *V, int64_t ldv, int j, const double2 *c, dou
 // symmetric tridiagonal eigensolver (implicit QL with Wilkinson shift, the
 // textbook method of Golub & Van Loan 8.3), FP64
 // ---------------------------------------------------------------------------
-int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z) {
+// Z receives rows `rows[0..nr-1]` of the eigenvector matrix, column-major
+// (Z[r + i*nr] = component rows[r] of eigenvector i); rows == null means all
+// n rows (nr = n).  The rotations act on each row independently, so tracking
+// a few rows costs O(n) per rotation sweep instead of O(n^2).
+int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z, int nr,
+                    const int *rows) {
+  if (!rows) nr = n;
   std::vector<double> e(n + 1, 0.0);
   for (int i = 0; i < n; i++) d[i] = alpha[i];
   for (int i = 0; i + 1 < n; i++) e[i] = beta[i];
-  for (int i = 0; i < n * n; i++) Z[i] = 0.0;
-  for (int i = 0; i < n; i++) Z[i + (size_t)i * n] = 1.0;
+  for (int i = 0; i < nr * n; i++) Z[i] = 0.0;
+  for (int r = 0; r < nr; r++) Z[r + (size_t)(rows ? rows[r] : r) * nr] = 1.0;
   const double eps = 2.220446049250313e-16;
   for (int l = 0; l < n; l++) {
     int iter = 0, m;
@@ -295,10 +335,10 @@ int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, d
           p = s * r;
           d[i + 1] = g + p;
           g = c * r - bb;
-          for (int k = 0; k < n; k++) {
-            double zf = Z[k + (size_t)(i + 1) * n];
-            Z[k + (size_t)(i + 1) * n] = s * Z[k + (size_t)i * n] + c * zf;
-            Z[k + (size_t)i * n] = c * Z[k + (size_t)i * n] - s * zf;
+          for (int k = 0; k < nr; k++) {
+            double zf = Z[k + (size_t)(i + 1) * nr];
+            Z[k + (size_t)(i + 1) * nr] = s * Z[k + (size_t)i * nr] + c * zf;
+            Z[k + (size_t)i * nr] = c * Z[k + (size_t)i * nr] - s * zf;
           }
         }
         if (r == 0.0 && i >= l) continue;
@@ -314,7 +354,7 @@ int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, d
       if (d[j] < d[k]) k = j;
     if (k != i) {
       std::swap(d[i], d[k]);
-      for (int q = 0; q < n; q++) std::swap(Z[q + (size_t)i * n], Z[q + (size_t)k * n]);
+      for (int q = 0; q < nr; q++) std::swap(Z[q + (size_t)i * nr], Z[q + (size_t)k * nr]);
     }
   }
   return EDK_OK;
@@ -400,11 +440,14 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
       rc = -br;
       break;
     }
-    if ((rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data()))) break;
+    // convergence needs only the last row of the eigenvector matrix
+    const int last = mm - 1;
+    if ((rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data(), 1, &last))) break;
     e0 = lam[0];
-    res = br ? 0.0 : std::fabs(beta[mm - 1] * Q[(mm - 1)]);
+    res = br ? 0.0 : std::fabs(beta[mm - 1] * Q[0]);
     if (br || (tol > 0 && res <= tol)) break;
   }
+  if (!rc && (refine || vec_out)) rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data(), mm, nullptr);
   if (!rc && (refine || vec_out)) {
     std::vector<double2> c(mm);
     for (int k = 0; k < mm; k++) c[k] = make_double2(Q[k], 0.0);
@@ -494,7 +537,8 @@ static int build_reduced(edk_krylov *c) {
   c->Q.assign((size_t)m * m, 0);
   c->Qe1.assign(m, 0);
   c->Qem.assign(m, 0);
-  if (int rc = edk_tridiag_eig(m, c->alpha.data(), c->beta.data(), c->lambda.data(), c->Q.data())) return rc;
+  if (int rc = edk_tridiag_eig(m, c->alpha.data(), c->beta.data(), c->lambda.data(), c->Q.data(), m, nullptr))
+    return rc;
   for (int k = 0; k < m; k++) {
     c->Qe1[k] = c->Q[0 + (size_t)k * m];
     c->Qem[k] = c->Q[(m - 1) + (size_t)k * m];
