@@ -1,0 +1,408 @@
+// edk_fast.h -- shared device code of the specialised gather kernels
+// (edk_fast.hip: one vector; edk_fast_blk.hip: blocks of vectors).
+//
+// edk_fast.hip -- specialised gather kernel for translation-invariant
+// two-site operators on 32-bit packed words.
+//
+// Applies when (checked on the host, else the generic kernel runs):
+//   - base 2, L <= 32, so a configuration is one 32-bit word;
+//   - the basis is TranslationParityBasis, TranslationalBasis (any k, unit
+//     cell a) or ProjectedBasis, built by the library (orbit norms known);
+//   - the operator is a sum of "families": for a fixed 2x2-site matrix M and
+//     site distance d, the terms M on [i, i+d mod L] for every i -- exactly
+//     what trans_inv_operator(M, [1, 1+d], B) builds (Operator.jl:102-108);
+//   - M is real.
+//
+// Row i with configuration s: for a family (M, d) let r = rotl(s, d); bit q of
+// (s, r) is the pair (site i, site i+d) of the term whose first site sits at
+// bit q.  The diagonal is sum_ab M[ab,ab] * popcount(sel_a(s) & sel_b(r));
+// every off-diagonal entry M[loc, c] acts at the bits of
+// sel(loc_a, s) & sel(loc_b, r), flipping (loc ^ c) at (q, q - d).  Targets
+// are processed eight at a time: canonicalise all eight, then issue their
+// directory reads, their in-bucket binary searches and their x gathers
+// together, so each lane keeps eight independent memory requests in flight.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "edk_matvec.h"
+
+#define FAST_MAXFAM 4
+#define FAST_MAXENT 12
+#define FAST_TPB 256
+#define FAST_TILE 4096
+
+struct FamEnt {
+  int locs;    // bit loc set: the entry acts where (first, second) site digits == loc
+  int flip;    // bit 1: flip first site, bit 0: flip second site
+  double val;  // M[loc, c], equal for every loc in locs
+};
+
+// positions (bits of the first site) whose local configuration is in locs
+__device__ __forceinline__ uint32_t ent_mask(int locs, uint32_t s, uint32_t ns, uint32_t r, uint32_t mask) {
+  uint32_t nr = ~r & mask, m = 0;
+  if (locs & 1) m |= ns & nr;
+  if (locs & 2) m |= ns & r;
+  if (locs & 4) m |= s & nr;
+  if (locs & 8) m |= s & r;
+  return m;
+}
+
+struct FastFam {
+  int d;
+  int nent;
+  double diag[4];  // M[ab, ab]
+  FamEnt ent[FAST_MAXENT];
+};
+
+struct FastTab {
+  int nfam;
+  FastFam fam[FAST_MAXFAM];
+};
+
+struct FastArgs {
+  int kind, L, A, len, kp, P;
+  uint32_t mask;
+  int ndiv;
+  int divs[16];  // divisors of len below len, ascending (orbit period candidates)
+  int batch;     // targets per lane handled together (4, 8 or 16)
+  int scan;      // bucket width searched with one wide scan (0: binary search only)
+  const FastTab *tab;
+  const double *Rtab;  // orbit norms (same table the enumerator used)
+  const double2 *ph;   // TranslationalBasis phases C[m]
+  const uint32_t *keys;  // D keys + padding of 0xffffffff
+  const uint32_t *dir;
+  int dshift;
+  uint32_t ndir;
+  uint32_t D;
+  const double *Rrow;
+  const uint16_t *perm;  // per tile of FAST_TILE rows: row order by target count (or null)
+  const uint2 *region;   // two-level directory: per region (key >> rshift) {base index, fine offset << 5 | shift}
+  int rshift;            // region width in key bits
+  const uint16_t *fine;  // per region: positions relative to base, one slot per 2^shift key values
+  const uint32_t *chunk; // balanced schedule: first row of each chunk (nchunk + 1 entries), or null
+  uint32_t nchunk;
+  const uint4 *htab;     // hash index: buckets of 8 (key, position) slots, or null
+  uint32_t hbuckets;     // number of buckets
+  int hprobe;            // longest probe sequence in buckets (>= 1)
+};
+
+__device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
+  if (L == 32) return __builtin_amdgcn_alignbit(s, s, sh);
+  uint32_t hi = s >> (32 - L), lo = s | (s << L);
+  return __builtin_amdgcn_alignbit(hi, lo, sh) & mask;
+}
+
+__device__ __forceinline__ uint32_t rev32(uint32_t s, int L) { return __brev(s) >> (32 - L); }
+
+// Canonicalise s'.  Returns false when the representative cannot be in the
+// table; otherwise rep and the real (cr) or complex (cr, ci) factor
+// conj(phase) * R_rep.  All loops have uniform trip counts (no per-lane
+// divergence): the minimum over a whole cycle of len rotations equals the
+// minimum over one period, and the first index attaining it lies in the
+// first period, as in the reference's loops that stop at the period.
+template <int KIND, int PH>
+__device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
+                                           double &ci) {
+  if (KIND == EDK_PROJECTED) {
+    rep = s;
+    cr = 1.0;
+    ci = 0.0;
+    return true;
+  }
+  const int sh = A.A;
+  int period = A.len;
+  for (int q = 0; q < A.ndiv; q++) {
+    int p = A.divs[q];
+    bool m = rotr32(s, p * sh, A.L, A.mask) == s;
+    period = (m && period == A.len) ? p : period;
+  }
+  if (KIND == EDK_TRANSLATIONAL) {
+    uint32_t minA = s, cur = s;
+    for (int i = 1; i < A.len; i++) {
+      cur = rotr32(cur, sh, A.L, A.mask);
+      minA = min(minA, cur);
+    }
+    rep = minA;
+    if (((int64_t)period * A.kp) % A.len) return false;  // check_momentum
+    double R = A.Rtab[period - 1];
+    if (PH != 0) {
+      int M = 0;
+      cur = s;
+      for (int i = 1; i < A.len; i++) {
+        cur = rotr32(cur, sh, A.L, A.mask);
+        M = (cur == minA && M == 0 && s != minA) ? i : M;
+      }
+      double2 c = A.ph[M];
+      cr = c.x * R;
+      ci = -c.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    return true;
+  }
+  // TranslationParityBasis
+  uint32_t rb = rev32(s, A.L);
+  uint32_t minA = s, minB = rb, cur = s, curB = rb;
+  for (int i = 1; i < A.len; i++) {
+    cur = rotr32(cur, sh, A.L, A.mask);
+    curB = rotr32(curB, sh, A.L, A.mask);
+    minA = min(minA, cur);
+    minB = min(minB, curB);
+  }
+  bool qp = (minA == minB);
+  double R = A.Rtab[(qp ? A.len : 0) + period - 1];
+  double sgn = 1.0;
+  if (PH != 0) {
+    // first index M with rot^M(s) (or reverse(rot^M(s))) at the minimum
+    // (TranslationalParityBasis.jl:207-231)
+    int Ma = -1, Mb = -1;
+    cur = s;
+    for (int i = 0; i < A.len; i++) {
+      Ma = (cur == minA && Ma < 0) ? i : Ma;
+      Mb = (rev32(cur, A.L) == minB && Mb < 0) ? i : Mb;
+      cur = rotr32(cur, sh, A.L, A.mask);
+    }
+    if (minB < minA) sgn = (double)A.P * ((A.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (A.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
+  }
+  rep = minA < minB ? minA : minB;
+  cr = sgn * R;
+  ci = 0.0;
+  return true;
+}
+
+// The same canonicaliser for L = 32 and unit cell 1, fully unrolled: each of
+// the 31 rotations is one v_alignbit, the minima fold into v_min3, the
+// orbit period (a power of two dividing 32) is read off the same rotations.
+template <int KIND, int PH>
+__device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
+                                             double &ci) {
+  if (KIND == EDK_PROJECTED) {
+    rep = s;
+    cr = 1.0;
+    ci = 0.0;
+    return true;
+  }
+  // rotations as a dependent chain (rotate by one each step): few live
+  // registers, so the batch of targets can be interleaved without spills
+  uint32_t minA = s, cur = s;
+#pragma unroll
+  for (int k = 0; k < 15; k++) {  // rotations 1..30 in pairs (v_min3)
+    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
+    minA = min(minA, min(c1, c2));
+    cur = c2;
+  }
+  minA = min(minA, __builtin_amdgcn_alignbit(cur, cur, 1));
+  int period = 32;
+#pragma unroll
+  for (int k = 16; k >= 1; k >>= 1) period = (__builtin_amdgcn_alignbit(s, s, k) == s) ? k : period;
+  if (KIND == EDK_TRANSLATIONAL) {
+    rep = minA;
+    if (((int64_t)period * A.kp) % 32) return false;  // check_momentum
+    double R = A.Rtab[period - 1];
+    if (PH != 0) {
+      int M = 0;
+      bool found = (s == minA);
+      cur = s;
+#pragma unroll
+      for (int k = 1; k < 32; k++) {
+        cur = __builtin_amdgcn_alignbit(cur, cur, 1);
+        M = (!found && cur == minA) ? k : M;
+        found = found || cur == minA;
+      }
+      double2 c = A.ph[M];
+      cr = c.x * R;
+      ci = -c.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    return true;
+  }
+  uint32_t rb = __brev(s);
+  uint32_t minB = rb;
+  cur = rb;
+#pragma unroll
+  for (int k = 0; k < 15; k++) {
+    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
+    minB = min(minB, min(c1, c2));
+    cur = c2;
+  }
+  minB = min(minB, __builtin_amdgcn_alignbit(cur, cur, 1));
+  bool qp = (minA == minB);
+  double R = A.Rtab[(qp ? 32 : 0) + period - 1];
+  double sgn = 1.0;
+  if (PH != 0) {
+    // first k with rot^k(s) == minA, resp. reverse(rot^k(s)) == minB
+    // (TranslationalParityBasis.jl:207-231)
+    int Ma = 0, Mb = 0;
+    bool fa = (s == minA), fb = (rb == minB);
+    cur = s;
+#pragma unroll
+    for (int k = 1; k < 32; k++) {
+      cur = __builtin_amdgcn_alignbit(cur, cur, 1);
+      uint32_t cb = __brev(cur);
+      Ma = (!fa && cur == minA) ? k : Ma;
+      fa = fa || cur == minA;
+      Mb = (!fb && cb == minB) ? k : Mb;
+      fb = fb || cb == minB;
+    }
+    if (minB < minA) sgn = (double)A.P * ((A.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (A.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
+  }
+  rep = minA < minB ? minA : minB;
+  cr = sgn * R;
+  ci = 0.0;
+  return true;
+}
+
+#define HASH_EMPTY 0xffffffffu
+__device__ __forceinline__ uint32_t hash_bucket(uint32_t key, uint32_t nb) {
+  return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
+}
+
+// One bucket of the hash index: 8 (key, position) slots in 64 bytes.
+// Returns the position, HASH_EMPTY when the bucket proves the key absent
+// (it has a free slot), or HASH_EMPTY - 1 when the next bucket must be read.
+__device__ __forceinline__ uint32_t hash_probe(const uint4 *b, uint32_t key) {
+  uint4 w0 = b[0], w1 = b[1], w2 = b[2], w3 = b[3];
+  uint32_t r = HASH_EMPTY - 1;
+  bool empty = false;
+#define HSLOT(K, V)                         \
+  r = ((K) == key && (V) != HASH_EMPTY) ? (V) : r; \
+  empty = empty || (V) == HASH_EMPTY;
+  HSLOT(w0.x, w0.y) HSLOT(w0.z, w0.w) HSLOT(w1.x, w1.y) HSLOT(w1.z, w1.w)
+  HSLOT(w2.x, w2.y) HSLOT(w2.z, w2.w) HSLOT(w3.x, w3.y) HSLOT(w3.z, w3.w)
+#undef HSLOT
+  return (r == HASH_EMPTY - 1 && empty) ? HASH_EMPTY : r;
+}
+
+// Position of key in the sorted table, or -1.  The directory bounds the
+// search to [lo, hi); a bucket of at most A.scan keys is searched with one
+// set of independent 16-byte loads (one memory round), a larger one by binary
+// search.
+__device__ __forceinline__ int64_t fast_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
+  if (hi <= lo) return -1;
+  if (hi - lo <= (uint32_t)A.scan) {
+    uint32_t base = lo & ~3u;
+    const uint4 *p = (const uint4 *)(A.keys + base);
+    int below = 0, eq = 0;
+    int nv = (int)((hi - base + 3) >> 2);
+#pragma unroll 4
+    for (int v = 0; v < nv; v++) {
+      uint4 w = p[v];
+      uint32_t idx = base + 4 * v;
+      below += (idx + 0 >= lo && idx + 0 < hi && w.x < key);
+      below += (idx + 1 >= lo && idx + 1 < hi && w.y < key);
+      below += (idx + 2 >= lo && idx + 2 < hi && w.z < key);
+      below += (idx + 3 >= lo && idx + 3 < hi && w.w < key);
+      eq += (idx + 0 >= lo && idx + 0 < hi && w.x == key);
+      eq += (idx + 1 >= lo && idx + 1 < hi && w.y == key);
+      eq += (idx + 2 >= lo && idx + 2 < hi && w.z == key);
+      eq += (idx + 3 >= lo && idx + 3 < hi && w.w == key);
+    }
+    return eq ? (int64_t)lo + below : -1;
+  }
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (A.keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < A.D && A.keys[lo] == key) ? (int64_t)lo : -1;
+}
+
+// Two-level directory: region r = key >> rshift gives the region's first table
+// index and its fine directory (sized to the region's population); the fine
+// bucket bounds the key to [lo, hi), usually at most four keys, which are read
+// with independent loads in one round.
+#define FINE_SCAN 4
+__device__ __forceinline__ void region_bounds(const FastArgs &A, uint32_t key, uint32_t &lo, uint32_t &hi) {
+  uint2 rg = A.region[key >> A.rshift];
+  uint32_t sh = rg.y & 31, foff = rg.y >> 5;
+  uint32_t fb = (key & ((1u << A.rshift) - 1)) >> sh;
+  lo = rg.x + A.fine[foff + fb];
+  hi = rg.x + A.fine[foff + fb + 1];
+}
+
+__device__ __forceinline__ uint32_t region_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
+  if (hi <= lo) return HASH_EMPTY;
+  if (hi - lo <= FINE_SCAN) {
+    uint32_t below = 0;
+    bool eq = false;
+#pragma unroll
+    for (int k = 0; k < FINE_SCAN; k++) {
+      uint32_t v = A.keys[lo + k];  // padded table: reads past the bucket are harmless
+      bool in = lo + k < hi;
+      below += (in && v < key);
+      eq = eq || (in && v == key);
+    }
+    return eq ? lo + below : HASH_EMPTY;
+  }
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint32_t v = A.keys[mid];
+    if (v == key) return mid;
+    if (v < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return HASH_EMPTY;
+}
+
+// ---------------------------------------------------------------------------
+// balanced kernel: one block per chunk of whole rows holding at most
+// BAL_CAP targets.  Phase 1: the row threads write their targets (and entry
+// numbers) into LDS in generator order.  Phase 2: every thread takes BAL_K
+// targets (j = tid + BAL_TPB q), canonicalises them, looks them up and gathers
+// x, all BAL_K independent so they overlap.  Phase 3: each row thread sums its
+// contiguous slice of contributions in order and writes y.
+// ---------------------------------------------------------------------------
+#define BAL_TPB 256
+#define BAL_K 4
+#define BAL_CAP (BAL_TPB * BAL_K)
+
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total, uint32_t *wsum) {
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t xx = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t yv = __shfl_up(xx, o, 64);
+    if (lane >= o) xx += yv;
+  }
+  if (lane == 63) wsum[wid] = xx;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int q = 0; q < BAL_TPB / 64; q++) {
+    if (q < wid) base += wsum[q];
+    tot += wsum[q];
+  }
+  *total = tot;
+  return base + xx - v;
+}
+
+// ---------------------------------------------------------------------------
+// host: eligibility and setup (cached on the operator)
+// ---------------------------------------------------------------------------
+struct FastState {
+  bool checked = false;
+  bool ok = false;
+  FastArgs args;
+  FastTab *d_tab = nullptr;
+  uint32_t *d_keys32 = nullptr;
+  uint32_t *d_dir = nullptr;
+  uint16_t *d_perm = nullptr;
+  uint4 *d_htab = nullptr;
+  uint2 *d_region = nullptr;
+  uint16_t *d_fine = nullptr;
+  uint32_t *d_chunk = nullptr;
+  bool cplx = false;
+  int tuned[2][9] = {};  // schedule per (dtype, block width 1..8), EDK_VARIANT_*
+};
+
+
+// block-of-vectors launch (edk_fast_blk.hip); nv in {2, 4, 8}
+bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, int nv, int acc, hipStream_t st);

@@ -1,354 +1,6 @@
-// edk_fast.hip -- specialised gather kernel for translation-invariant
-// two-site operators on 32-bit packed words.
-//
-// Applies when (checked on the host, else the generic kernel runs):
-//   - base 2, L <= 32, so a configuration is one 32-bit word;
-//   - the basis is TranslationParityBasis, TranslationalBasis (any k, unit
-//     cell a) or ProjectedBasis, built by the library (orbit norms known);
-//   - the operator is a sum of "families": for a fixed 2x2-site matrix M and
-//     site distance d, the terms M on [i, i+d mod L] for every i -- exactly
-//     what trans_inv_operator(M, [1, 1+d], B) builds (Operator.jl:102-108);
-//   - M is real.
-//
-// Row i with configuration s: for a family (M, d) let r = rotl(s, d); bit q of
-// (s, r) is the pair (site i, site i+d) of the term whose first site sits at
-// bit q.  The diagonal is sum_ab M[ab,ab] * popcount(sel_a(s) & sel_b(r));
-// every off-diagonal entry M[loc, c] acts at the bits of
-// sel(loc_a, s) & sel(loc_b, r), flipping (loc ^ c) at (q, q - d).  Targets
-// are processed eight at a time: canonicalise all eight, then issue their
-// directory reads, their in-bucket binary searches and their x gathers
-// together, so each lane keeps eight independent memory requests in flight.
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <vector>
-
-#include "edk_matvec.h"
-
-#define FAST_MAXFAM 4
-#define FAST_MAXENT 12
-#define FAST_TPB 256
-#define FAST_TILE 4096
-
-struct FamEnt {
-  int locs;    // bit loc set: the entry acts where (first, second) site digits == loc
-  int flip;    // bit 1: flip first site, bit 0: flip second site
-  double val;  // M[loc, c], equal for every loc in locs
-};
-
-// positions (bits of the first site) whose local configuration is in locs
-__device__ __forceinline__ uint32_t ent_mask(int locs, uint32_t s, uint32_t ns, uint32_t r, uint32_t mask) {
-  uint32_t nr = ~r & mask, m = 0;
-  if (locs & 1) m |= ns & nr;
-  if (locs & 2) m |= ns & r;
-  if (locs & 4) m |= s & nr;
-  if (locs & 8) m |= s & r;
-  return m;
-}
-
-struct FastFam {
-  int d;
-  int nent;
-  double diag[4];  // M[ab, ab]
-  FamEnt ent[FAST_MAXENT];
-};
-
-struct FastTab {
-  int nfam;
-  FastFam fam[FAST_MAXFAM];
-};
-
-struct FastArgs {
-  int kind, L, A, len, kp, P;
-  uint32_t mask;
-  int ndiv;
-  int divs[16];  // divisors of len below len, ascending (orbit period candidates)
-  int batch;     // targets per lane handled together (4, 8 or 16)
-  int scan;      // bucket width searched with one wide scan (0: binary search only)
-  const FastTab *tab;
-  const double *Rtab;  // orbit norms (same table the enumerator used)
-  const double2 *ph;   // TranslationalBasis phases C[m]
-  const uint32_t *keys;  // D keys + padding of 0xffffffff
-  const uint32_t *dir;
-  int dshift;
-  uint32_t ndir;
-  uint32_t D;
-  const double *Rrow;
-  const uint16_t *perm;  // per tile of FAST_TILE rows: row order by target count (or null)
-  const uint2 *region;   // two-level directory: per region (key >> rshift) {base index, fine offset << 5 | shift}
-  int rshift;            // region width in key bits
-  const uint16_t *fine;  // per region: positions relative to base, one slot per 2^shift key values
-  const uint32_t *chunk; // balanced schedule: first row of each chunk (nchunk + 1 entries), or null
-  uint32_t nchunk;
-  const uint4 *htab;     // hash index: buckets of 8 (key, position) slots, or null
-  uint32_t hbuckets;     // number of buckets
-  int hprobe;            // longest probe sequence in buckets (>= 1)
-};
-
-__device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
-  if (L == 32) return __builtin_amdgcn_alignbit(s, s, sh);
-  uint32_t hi = s >> (32 - L), lo = s | (s << L);
-  return __builtin_amdgcn_alignbit(hi, lo, sh) & mask;
-}
-
-__device__ __forceinline__ uint32_t rev32(uint32_t s, int L) { return __brev(s) >> (32 - L); }
-
-// Canonicalise s'.  Returns false when the representative cannot be in the
-// table; otherwise rep and the real (cr) or complex (cr, ci) factor
-// conj(phase) * R_rep.  All loops have uniform trip counts (no per-lane
-// divergence): the minimum over a whole cycle of len rotations equals the
-// minimum over one period, and the first index attaining it lies in the
-// first period, as in the reference's loops that stop at the period.
-template <int KIND, int PH>
-__device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
-                                           double &ci) {
-  if (KIND == EDK_PROJECTED) {
-    rep = s;
-    cr = 1.0;
-    ci = 0.0;
-    return true;
-  }
-  const int sh = A.A;
-  int period = A.len;
-  for (int q = 0; q < A.ndiv; q++) {
-    int p = A.divs[q];
-    bool m = rotr32(s, p * sh, A.L, A.mask) == s;
-    period = (m && period == A.len) ? p : period;
-  }
-  if (KIND == EDK_TRANSLATIONAL) {
-    uint32_t minA = s, cur = s;
-    for (int i = 1; i < A.len; i++) {
-      cur = rotr32(cur, sh, A.L, A.mask);
-      minA = min(minA, cur);
-    }
-    rep = minA;
-    if (((int64_t)period * A.kp) % A.len) return false;  // check_momentum
-    double R = A.Rtab[period - 1];
-    if (PH != 0) {
-      int M = 0;
-      cur = s;
-      for (int i = 1; i < A.len; i++) {
-        cur = rotr32(cur, sh, A.L, A.mask);
-        M = (cur == minA && M == 0 && s != minA) ? i : M;
-      }
-      double2 c = A.ph[M];
-      cr = c.x * R;
-      ci = -c.y * R;
-    } else {
-      cr = R;
-      ci = 0.0;
-    }
-    return true;
-  }
-  // TranslationParityBasis
-  uint32_t rb = rev32(s, A.L);
-  uint32_t minA = s, minB = rb, cur = s, curB = rb;
-  for (int i = 1; i < A.len; i++) {
-    cur = rotr32(cur, sh, A.L, A.mask);
-    curB = rotr32(curB, sh, A.L, A.mask);
-    minA = min(minA, cur);
-    minB = min(minB, curB);
-  }
-  bool qp = (minA == minB);
-  double R = A.Rtab[(qp ? A.len : 0) + period - 1];
-  double sgn = 1.0;
-  if (PH != 0) {
-    // first index M with rot^M(s) (or reverse(rot^M(s))) at the minimum
-    // (TranslationalParityBasis.jl:207-231)
-    int Ma = -1, Mb = -1;
-    cur = s;
-    for (int i = 0; i < A.len; i++) {
-      Ma = (cur == minA && Ma < 0) ? i : Ma;
-      Mb = (rev32(cur, A.L) == minB && Mb < 0) ? i : Mb;
-      cur = rotr32(cur, sh, A.L, A.mask);
-    }
-    if (minB < minA) sgn = (double)A.P * ((A.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
-    else sgn = (A.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
-  }
-  rep = minA < minB ? minA : minB;
-  cr = sgn * R;
-  ci = 0.0;
-  return true;
-}
-
-// The same canonicaliser for L = 32 and unit cell 1, fully unrolled: each of
-// the 31 rotations is one v_alignbit, the minima fold into v_min3, the
-// orbit period (a power of two dividing 32) is read off the same rotations.
-template <int KIND, int PH>
-__device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
-                                             double &ci) {
-  if (KIND == EDK_PROJECTED) {
-    rep = s;
-    cr = 1.0;
-    ci = 0.0;
-    return true;
-  }
-  // rotations as a dependent chain (rotate by one each step): few live
-  // registers, so the batch of targets can be interleaved without spills
-  uint32_t minA = s, cur = s;
-#pragma unroll
-  for (int k = 0; k < 15; k++) {  // rotations 1..30 in pairs (v_min3)
-    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
-    minA = min(minA, min(c1, c2));
-    cur = c2;
-  }
-  minA = min(minA, __builtin_amdgcn_alignbit(cur, cur, 1));
-  int period = 32;
-#pragma unroll
-  for (int k = 16; k >= 1; k >>= 1) period = (__builtin_amdgcn_alignbit(s, s, k) == s) ? k : period;
-  if (KIND == EDK_TRANSLATIONAL) {
-    rep = minA;
-    if (((int64_t)period * A.kp) % 32) return false;  // check_momentum
-    double R = A.Rtab[period - 1];
-    if (PH != 0) {
-      int M = 0;
-      bool found = (s == minA);
-      cur = s;
-#pragma unroll
-      for (int k = 1; k < 32; k++) {
-        cur = __builtin_amdgcn_alignbit(cur, cur, 1);
-        M = (!found && cur == minA) ? k : M;
-        found = found || cur == minA;
-      }
-      double2 c = A.ph[M];
-      cr = c.x * R;
-      ci = -c.y * R;
-    } else {
-      cr = R;
-      ci = 0.0;
-    }
-    return true;
-  }
-  uint32_t rb = __brev(s);
-  uint32_t minB = rb;
-  cur = rb;
-#pragma unroll
-  for (int k = 0; k < 15; k++) {
-    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
-    minB = min(minB, min(c1, c2));
-    cur = c2;
-  }
-  minB = min(minB, __builtin_amdgcn_alignbit(cur, cur, 1));
-  bool qp = (minA == minB);
-  double R = A.Rtab[(qp ? 32 : 0) + period - 1];
-  double sgn = 1.0;
-  if (PH != 0) {
-    // first k with rot^k(s) == minA, resp. reverse(rot^k(s)) == minB
-    // (TranslationalParityBasis.jl:207-231)
-    int Ma = 0, Mb = 0;
-    bool fa = (s == minA), fb = (rb == minB);
-    cur = s;
-#pragma unroll
-    for (int k = 1; k < 32; k++) {
-      cur = __builtin_amdgcn_alignbit(cur, cur, 1);
-      uint32_t cb = __brev(cur);
-      Ma = (!fa && cur == minA) ? k : Ma;
-      fa = fa || cur == minA;
-      Mb = (!fb && cb == minB) ? k : Mb;
-      fb = fb || cb == minB;
-    }
-    if (minB < minA) sgn = (double)A.P * ((A.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
-    else sgn = (A.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
-  }
-  rep = minA < minB ? minA : minB;
-  cr = sgn * R;
-  ci = 0.0;
-  return true;
-}
-
-#define HASH_EMPTY 0xffffffffu
-__device__ __forceinline__ uint32_t hash_bucket(uint32_t key, uint32_t nb) {
-  return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
-}
-
-// One bucket of the hash index: 8 (key, position) slots in 64 bytes.
-// Returns the position, HASH_EMPTY when the bucket proves the key absent
-// (it has a free slot), or HASH_EMPTY - 1 when the next bucket must be read.
-__device__ __forceinline__ uint32_t hash_probe(const uint4 *b, uint32_t key) {
-  uint4 w0 = b[0], w1 = b[1], w2 = b[2], w3 = b[3];
-  uint32_t r = HASH_EMPTY - 1;
-  bool empty = false;
-#define HSLOT(K, V)                         \
-  r = ((K) == key && (V) != HASH_EMPTY) ? (V) : r; \
-  empty = empty || (V) == HASH_EMPTY;
-  HSLOT(w0.x, w0.y) HSLOT(w0.z, w0.w) HSLOT(w1.x, w1.y) HSLOT(w1.z, w1.w)
-  HSLOT(w2.x, w2.y) HSLOT(w2.z, w2.w) HSLOT(w3.x, w3.y) HSLOT(w3.z, w3.w)
-#undef HSLOT
-  return (r == HASH_EMPTY - 1 && empty) ? HASH_EMPTY : r;
-}
-
-// Position of key in the sorted table, or -1.  The directory bounds the
-// search to [lo, hi); a bucket of at most A.scan keys is searched with one
-// set of independent 16-byte loads (one memory round), a larger one by binary
-// search.
-__device__ __forceinline__ int64_t fast_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
-  if (hi <= lo) return -1;
-  if (hi - lo <= (uint32_t)A.scan) {
-    uint32_t base = lo & ~3u;
-    const uint4 *p = (const uint4 *)(A.keys + base);
-    int below = 0, eq = 0;
-    int nv = (int)((hi - base + 3) >> 2);
-#pragma unroll 4
-    for (int v = 0; v < nv; v++) {
-      uint4 w = p[v];
-      uint32_t idx = base + 4 * v;
-      below += (idx + 0 >= lo && idx + 0 < hi && w.x < key);
-      below += (idx + 1 >= lo && idx + 1 < hi && w.y < key);
-      below += (idx + 2 >= lo && idx + 2 < hi && w.z < key);
-      below += (idx + 3 >= lo && idx + 3 < hi && w.w < key);
-      eq += (idx + 0 >= lo && idx + 0 < hi && w.x == key);
-      eq += (idx + 1 >= lo && idx + 1 < hi && w.y == key);
-      eq += (idx + 2 >= lo && idx + 2 < hi && w.z == key);
-      eq += (idx + 3 >= lo && idx + 3 < hi && w.w == key);
-    }
-    return eq ? (int64_t)lo + below : -1;
-  }
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (A.keys[mid] < key) lo = mid + 1;
-    else hi = mid;
-  }
-  return (lo < A.D && A.keys[lo] == key) ? (int64_t)lo : -1;
-}
-
-// Two-level directory: region r = key >> rshift gives the region's first table
-// index and its fine directory (sized to the region's population); the fine
-// bucket bounds the key to [lo, hi), usually at most four keys, which are read
-// with independent loads in one round.
-#define FINE_SCAN 4
-__device__ __forceinline__ void region_bounds(const FastArgs &A, uint32_t key, uint32_t &lo, uint32_t &hi) {
-  uint2 rg = A.region[key >> A.rshift];
-  uint32_t sh = rg.y & 31, foff = rg.y >> 5;
-  uint32_t fb = (key & ((1u << A.rshift) - 1)) >> sh;
-  lo = rg.x + A.fine[foff + fb];
-  hi = rg.x + A.fine[foff + fb + 1];
-}
-
-__device__ __forceinline__ uint32_t region_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
-  if (hi <= lo) return HASH_EMPTY;
-  if (hi - lo <= FINE_SCAN) {
-    uint32_t below = 0;
-    bool eq = false;
-#pragma unroll
-    for (int k = 0; k < FINE_SCAN; k++) {
-      uint32_t v = A.keys[lo + k];  // padded table: reads past the bucket are harmless
-      bool in = lo + k < hi;
-      below += (in && v < key);
-      eq = eq || (in && v == key);
-    }
-    return eq ? lo + below : HASH_EMPTY;
-  }
-  while (lo < hi) {
-    uint32_t mid = (lo + hi) >> 1;
-    uint32_t v = A.keys[mid];
-    if (v == key) return mid;
-    if (v < key) lo = mid + 1;
-    else hi = mid;
-  }
-  return HASH_EMPTY;
-}
+// edk_fast.hip -- specialised gather kernels for one vector, and the host
+// setup they share with the block kernels (edk_fast.h describes the method).
+#include "edk_fast.h"
 
 template <int KIND, int PH, int NB, int LT, typename VT, typename ST>
 __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
@@ -478,36 +130,6 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
     out.y = yi;
   }
   y[i] = out;
-}
-
-// ---------------------------------------------------------------------------
-// balanced kernel: one block per chunk of whole rows holding at most
-// BAL_CAP targets.  Phase 1: the row threads write their targets (and entry
-// numbers) into LDS in generator order.  Phase 2: every thread takes BAL_K
-// targets (j = tid + BAL_TPB q), canonicalises them, looks them up and gathers
-// x, all BAL_K independent so they overlap.  Phase 3: each row thread sums its
-// contiguous slice of contributions in order and writes y.
-// ---------------------------------------------------------------------------
-#define BAL_TPB 256
-#define BAL_K 4
-#define BAL_CAP (BAL_TPB * BAL_K)
-
-__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total, uint32_t *wsum) {
-  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t xx = v;
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t yv = __shfl_up(xx, o, 64);
-    if (lane >= o) xx += yv;
-  }
-  if (lane == 63) wsum[wid] = xx;
-  __syncthreads();
-  uint32_t base = 0, tot = 0;
-  for (int q = 0; q < BAL_TPB / 64; q++) {
-    if (q < wid) base += wsum[q];
-    tot += wsum[q];
-  }
-  *total = tot;
-  return base + xx - v;
 }
 
 template <int KIND, int PH, int LT, typename VT, typename ST>
@@ -651,26 +273,9 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__
   }
 }
 
-// ---------------------------------------------------------------------------
-// host: eligibility and setup (cached on the operator)
-// ---------------------------------------------------------------------------
-struct FastState {
-  bool checked = false;
-  bool ok = false;
-  FastArgs args;
-  FastTab *d_tab = nullptr;
-  uint32_t *d_keys32 = nullptr;
-  uint32_t *d_dir = nullptr;
-  uint16_t *d_perm = nullptr;
-  uint4 *d_htab = nullptr;
-  uint2 *d_region = nullptr;
-  uint16_t *d_fine = nullptr;
-  uint32_t *d_chunk = nullptr;
-  bool cplx = false;
-};
-
 // number of off-diagonal targets the generator produces for each row
-__global__ void k_row_targets(FastArgs A, uint8_t *cnt) {
+template <typename CT>
+__global__ void k_row_targets(FastArgs A, CT *cnt) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.D) return;
   const FastTab &T = *A.tab;
@@ -680,7 +285,7 @@ __global__ void k_row_targets(FastArgs A, uint8_t *cnt) {
     uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
     for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
   }
-  cnt[i] = (uint8_t)min(n, 255);
+  cnt[i] = (CT)min(n, (int)(CT)~(CT)0);
 }
 
 __global__ void k_keys32(const uint64_t *k64, int64_t D, uint32_t *k32) {
@@ -953,7 +558,7 @@ static bool fast_setup(edk_op *op) {
     std::vector<uint8_t> cnt(b->D);
     std::vector<uint16_t> perm(b->D);
     if (hipMalloc((void **)&d_cnt, b->D) != hipSuccess) return false;
-    k_row_targets<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
+    k_row_targets<uint8_t><<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
     bool ok = hipMemcpyAsync(cnt.data(), d_cnt, b->D, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
               hipStreamSynchronize(ctx->stream) == hipSuccess;
     (void)hipFree(d_cnt);
@@ -977,11 +582,13 @@ static bool fast_setup(edk_op *op) {
   A.nchunk = 0;
   const char *sch = getenv("EDK_FAST_SCHEDULE");
   if (A.region && (!sch || strcmp(sch, "balanced") == 0)) {
-    uint8_t *d_cnt = nullptr;
-    std::vector<uint8_t> cnt(b->D);
-    if (hipMalloc((void **)&d_cnt, b->D) != hipSuccess) return false;
-    k_row_targets<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
-    bool okc = hipMemcpyAsync(cnt.data(), d_cnt, b->D, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+    // 16-bit counts: at most 3 targets per site, family and row (L <= 32)
+    uint16_t *d_cnt = nullptr;
+    std::vector<uint16_t> cnt(b->D);
+    if (hipMalloc((void **)&d_cnt, b->D * sizeof(uint16_t)) != hipSuccess) return false;
+    k_row_targets<uint16_t><<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
+    bool okc = hipMemcpyAsync(cnt.data(), d_cnt, b->D * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream) ==
+                   hipSuccess &&
                hipStreamSynchronize(ctx->stream) == hipSuccess;
     (void)hipFree(d_cnt);
     if (!okc) return false;
@@ -1054,12 +661,28 @@ static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, in
   else launch_fast_lt<KIND, PH, 0>(A, dtype, y, x, acc, st);
 }
 
-bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
-  if (nb != 1 || op->disable_fast) return false;
-  if (!fast_setup(op)) return false;
+bool edk_matvec_prepare(edk_op *op) { return !op->disable_fast && fast_setup(op); }
+
+// Launch one schedule: EDK_VARIANT_TILE is the balanced chunk kernel (one
+// block per chunk of whole rows), EDK_VARIANT_LANE{2,4,8} the thread-per-row
+// kernel with 2/4/8 targets in flight per lane.  Blocks of vectors have the
+// tile form only.  false: the variant does not apply (the caller falls back
+// to the generic kernel).
+static bool launch_variant(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate,
+                           hipStream_t st, int var) {
+  if (var == EDK_VARIANT_GENERIC) return false;
   FastState *S = (FastState *)op->fast;
-  const FastArgs &A = S->args;
-  hipStream_t st = op->ctx->stream;
+  FastArgs A = S->args;
+  if (nb != 1) {
+    if (nb > 8 || var > EDK_VARIANT_TILE) return false;
+    if (!edk_fast_launch_blk(A, dtype, y, x, (int)nb, accumulate, st)) return false;
+    return hipGetLastError() == hipSuccess;
+  }
+  if (var == EDK_VARIANT_TILE && !A.chunk) return false;
+  if (var >= EDK_VARIANT_LANE2) {
+    A.chunk = nullptr;
+    A.batch = var == EDK_VARIANT_LANE2 ? 2 : (var == EDK_VARIANT_LANE4 ? 4 : 8);
+  }
   // phase mode: 0 all coefficients real and positive, 1 real signs, 2 complex
   switch (A.kind) {
     case EDK_TRANSLATION_PARITY:
@@ -1075,4 +698,82 @@ bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, 
     default: return false;
   }
   return hipGetLastError() == hipSuccess;
+}
+
+bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st) {
+  if (op->disable_fast) return false;
+  if (!fast_setup(op)) return false;
+  FastState *S = (FastState *)op->fast;
+  const int var = (nb >= 1 && nb <= 8) ? S->tuned[dtype == EDK_C64 ? 0 : 1][nb] : EDK_VARIANT_AUTO;
+  return launch_variant(op, dtype, y, x, nb, accumulate, st, var);
+}
+
+int edk_fast_set_variant(edk_op *op, int dtype, int64_t nb, int variant) {
+  if (nb < 1 || nb > 8 || variant < EDK_VARIANT_AUTO || variant > EDK_VARIANT_LANE8)
+    return edk_fail(EDK_ERR_ARG, "variant or block width out of range");
+  if (variant >= EDK_VARIANT_LANE2 && nb != 1)
+    return edk_fail(EDK_ERR_ARG, "lane schedules apply to one vector only");
+  FastState *S = fast_state(op);
+  if (variant != EDK_VARIANT_AUTO && variant != EDK_VARIANT_GENERIC && !edk_matvec_prepare(op))
+    return edk_fail(EDK_ERR_UNSUPPORTED, "operator/basis not eligible for the specialised kernels");
+  S->tuned[dtype == EDK_C64 ? 0 : 1][nb] = variant;
+  return EDK_OK;
+}
+
+// Time every applicable schedule on scratch vectors (reps launches each, HIP
+// events on the context stream) and keep the fastest for (dtype, nb).
+int edk_fast_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant, double *ms_out) {
+  if (nb < 1 || nb > 8) return edk_fail(EDK_ERR_ARG, "autotune: block width must be 1..8");
+  if (reps < 1) reps = 1;
+  edk_ctx *ctx = op->ctx;
+  const int64_t D = op->basis->D;
+  const size_t bytes = (size_t)std::max<int64_t>(D, 1) * nb * (dtype == EDK_C64 ? 16 : 8);
+  std::vector<int> cand = {EDK_VARIANT_GENERIC};
+  if (edk_matvec_prepare(op)) {
+    FastState *S = (FastState *)op->fast;
+    if (S->args.chunk) cand.push_back(EDK_VARIANT_TILE);
+    if (nb == 1) {
+      cand.push_back(EDK_VARIANT_LANE2);
+      cand.push_back(EDK_VARIANT_LANE4);
+      cand.push_back(EDK_VARIANT_LANE8);
+    }
+  }
+  void *x = nullptr, *y = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipMalloc(&x, bytes) != hipSuccess || hipMalloc(&y, bytes) != hipSuccess) {
+    (void)hipFree(x);
+    return edk_fail(EDK_ERR_OOM, "autotune scratch vectors");
+  }
+  int rc = EDK_OK;
+  if (hipMemsetAsync(x, 0, bytes, ctx->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    rc = edk_fail(EDK_ERR_HIP, "autotune setup");
+  FastState *S = fast_state(op);
+  const int di = dtype == EDK_C64 ? 0 : 1;
+  int best = EDK_VARIANT_GENERIC;
+  double best_ms = 1e300;
+  for (size_t c = 0; !rc && c < cand.size(); c++) {
+    S->tuned[di][nb] = cand[c];
+    if ((rc = edk_matvec_launch(op, dtype, y, x, nb, 0))) break;
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = edk_fail(EDK_ERR_HIP, "autotune event");
+    for (int r = 0; !rc && r < reps; r++) rc = edk_matvec_launch(op, dtype, y, x, nb, 0);
+    float ms = 0;
+    if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = edk_fail(EDK_ERR_HIP, "autotune timing");
+    if (!rc && ms / reps < best_ms) {
+      best_ms = ms / reps;
+      best = cand[c];
+    }
+  }
+  S->tuned[di][nb] = rc ? EDK_VARIANT_AUTO : best;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (rc) return rc;
+  if (variant) *variant = best;
+  if (ms_out) *ms_out = best_ms;
+  return EDK_OK;
 }

@@ -56,5 +56,14 @@ __device__ __forceinline__ uint64_t edk_content_to_packed_d(uint64_t c, int L, i
 
 // Specialised kernels; returns true when it handled the call.
 void edk_fast_release(edk_op *op);
-bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate);
-int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate);
+bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);
+// schedule choice per (dtype, block width): forced, or timed by the autotuner
+int edk_fast_set_variant(edk_op *op, int dtype, int64_t nb, int variant);
+int edk_fast_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant, double *ms);
+// one-time setup of the specialised kernels (allocates; run before capture)
+bool edk_matvec_prepare(edk_op *op);
+// y = (accumulate ? y : 0) + H x on stream st (no allocation, no synchronisation)
+int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);
+inline int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
+  return edk_matvec_launch(op, dtype, y, x, nb, accumulate, op->ctx->stream);
+}

@@ -310,19 +310,16 @@ extern "C" int edk_operator_count_offdiag(edk_op *op, int64_t *nnz) {
   return EDK_OK;
 }
 
-int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {
-  edk_ctx *ctx = op->ctx;
+int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st) {
   MatvecArgs a = op->args();
   if (a.D == 0) return EDK_OK;
   unsigned grid = (unsigned)((a.D + 255) / 256);
-  if (edk_matvec_fast(op, dtype, y, x, nb, accumulate)) return EDK_OK;
+  if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   for (int64_t c = 0; c < nb; c++) {
     if (dtype == EDK_C64)
-      k_matvec_generic<double2, double><<<grid, 256, 0, ctx->stream>>>(a, (const double2 *)x, (double2 *)y, nb, c,
-                                                                       accumulate);
+      k_matvec_generic<double2, double><<<grid, 256, 0, st>>>(a, (const double2 *)x, (double2 *)y, nb, c, accumulate);
     else
-      k_matvec_generic<float2, float><<<grid, 256, 0, ctx->stream>>>(a, (const float2 *)x, (float2 *)y, nb, c,
-                                                                     accumulate);
+      k_matvec_generic<float2, float><<<grid, 256, 0, st>>>(a, (const float2 *)x, (float2 *)y, nb, c, accumulate);
   }
   EDK_HIP(hipGetLastError());
   return EDK_OK;

@@ -8,6 +8,7 @@ KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, plus the new
 lanczos_ground.
 """
 from ._lib import EdkError, lib  # noqa: F401
+from .batch import MatvecBatch  # noqa: F401
 from .basis import (AbelianBasis, AbstractBasis, ProjectedBasis, TensorBasis,  # noqa: F401
                     TranslationalBasis, TranslationParityBasis, basis, productstate)
 from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401

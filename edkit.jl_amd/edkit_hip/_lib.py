@@ -13,6 +13,9 @@ LIB_PATH = os.path.join(_HERE, "libedkit_hip.so")
 EDK_OK = 0
 EDK_TENSOR, EDK_PROJECTED, EDK_TRANSLATIONAL, EDK_TRANSLATION_PARITY, EDK_ABELIAN = range(5)
 EDK_C64, EDK_C32 = 0, 1
+(EDK_VARIANT_AUTO, EDK_VARIANT_GENERIC, EDK_VARIANT_TILE, EDK_VARIANT_LANE2, EDK_VARIANT_LANE4,
+ EDK_VARIANT_LANE8) = range(6)
+VARIANT_NAMES = {0: "auto", 1: "generic", 2: "tile", 3: "lane2", 4: "lane4", 5: "lane8"}
 
 # every symbol include/edkit_hip.h declares
 SYMBOLS = [
@@ -23,6 +26,8 @@ SYMBOLS = [
     "edk_basis_download", "edk_basis_index", "edk_basis_destroy",
     "edk_operator_create", "edk_operator_destroy", "edk_operator_count_offdiag",
     "edk_matvec", "edk_matvec_host",
+    "edk_operator_set_variant", "edk_operator_autotune",
+    "edk_batch_create", "edk_batch_launch", "edk_batch_destroy",
     "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2",
     "edk_lanczos_ground",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
@@ -44,6 +49,11 @@ class BasisSpec(C.Structure):
                 ("has_k", C.c_int32), ("has_p", C.c_int32), ("has_z", C.c_int32),
                 ("ncustom", C.c_int32), ("perms", C.c_void_p), ("ks", C.c_void_p),
                 ("invs", C.c_void_p)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("op", C.c_void_p), ("x", C.c_void_p), ("y", C.c_void_p), ("nb", C.c_int64),
+                ("accumulate", C.c_int32)]
 
 
 class KrylovParams(C.Structure):
@@ -98,6 +108,11 @@ def lib():
         "edk_operator_count_offdiag": (I32, [P, C.POINTER(C.c_int64)]),
         "edk_matvec": (I32, [P, I32, P, P, I64, I32]),
         "edk_matvec_host": (I32, [P, I32, P, P, I64, I32]),
+        "edk_operator_set_variant": (I32, [P, I32, I64, I32]),
+        "edk_operator_autotune": (I32, [P, I32, I64, I32, C.POINTER(C.c_int), C.POINTER(D)]),
+        "edk_batch_create": (I32, [P, I32, C.POINTER(BatchItem), I64, I32, PP]),
+        "edk_batch_launch": (I32, [P]),
+        "edk_batch_destroy": (I32, [P]),
         "edk_vec_scale": (I32, [P, I32, P, P, D, I64]),
         "edk_vec_axpy": (I32, [P, I32, P, P, D, D, I64]),
         "edk_vec_dot": (I32, [P, I32, P, P, I64, C.POINTER(D), C.POINTER(D)]),

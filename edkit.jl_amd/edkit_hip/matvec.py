@@ -28,6 +28,17 @@ class DeviceOperator:
                                         rv.ctypes.data, nz.ctypes.data, C.byref(h)))
         self.h = h
 
+    def set_variant(self, dtype, nb, variant):
+        """Force a schedule (EDK_VARIANT_*) for this dtype and block width."""
+        check(lib().edk_operator_set_variant(self.h, dtype_code(np.dtype(dtype)), int(nb), int(variant)))
+
+    def autotune(self, dtype=np.complex128, nb=1, reps=5):
+        """Time the applicable schedules and keep the fastest; returns (variant, ms)."""
+        v, ms = C.c_int(), C.c_double()
+        check(lib().edk_operator_autotune(self.h, dtype_code(np.dtype(dtype)), int(nb), int(reps),
+                                          C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
     def count_offdiag(self):
         n = C.c_int64()
         check(lib().edk_operator_count_offdiag(self.h, C.byref(n)))

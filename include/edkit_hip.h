@@ -150,6 +150,46 @@ int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
 int edk_matvec_host(edk_op *op, int dtype, void *y_host, const void *x_host,
                     int64_t nb, int accumulate);
 
+/* ---- schedules and batching ------------------------------------------------ */
+/* No counterpart in the reference (its mul! has one CPU loop).  Schedules of
+ * the specialised gather kernel, chosen per operator, dtype and block width:
+ *   EDK_VARIANT_AUTO     library default (tile where it applies)
+ *   EDK_VARIANT_GENERIC  the generic kernel (any basis and operator)
+ *   EDK_VARIANT_TILE     one workgroup per chunk of whole rows (<= 1024 targets),
+ *                        targets spread over all lanes; blocks of 2/4/8 vectors
+ *   EDK_VARIANT_LANE2/4/8  one lane per row, 2/4/8 targets in flight (nb = 1) */
+#define EDK_VARIANT_AUTO 0
+#define EDK_VARIANT_GENERIC 1
+#define EDK_VARIANT_TILE 2
+#define EDK_VARIANT_LANE2 3
+#define EDK_VARIANT_LANE4 4
+#define EDK_VARIANT_LANE8 5
+int edk_operator_set_variant(edk_op *op, int dtype, int64_t nb, int variant);
+/* Time each applicable schedule (reps launches on scratch vectors) and keep
+ * the fastest for later edk_matvec calls with this dtype and nb. */
+int edk_operator_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant, double *ms);
+
+/* Many sectors and block widths in one launch: the matvecs of n items are
+ * recorded once as a HIP graph (y = (accumulate ? y : 0) + H x per item,
+ * replacing a loop of mul!(Y_s, H_s, X_s) over sectors, Operator.jl:322-328).
+ * Items may use different operators/bases of one context; with streams > 1
+ * they are spread over that many graph branches, balanced by D * nb, so
+ * independent sectors run concurrently (streams <= 0: one branch when every
+ * item has D * nb >= 2^20, else 8).  The graph keeps the pointers: the
+ * vectors must stay allocated while the batch is used. */
+typedef struct {
+  edk_op *op;
+  const void *x; /* device, D * nb row-interleaved */
+  void *y;       /* device, D * nb row-interleaved, must not alias x */
+  int64_t nb;
+  int32_t accumulate;
+} edk_batch_item;
+typedef struct edk_batch edk_batch;
+int edk_batch_create(edk_ctx *ctx, int dtype, const edk_batch_item *items, int64_t n, int streams,
+                     edk_batch **out);
+int edk_batch_launch(edk_batch *b); /* one graph launch on the context stream, asynchronous */
+int edk_batch_destroy(edk_batch *b);
+
 /* ---- device vector kernels ------------------------------------------------- */
 /* The BLAS-1 steps the Krylov loops run inline in the reference
  * (TimeEvolution.jl:261-303: w -= beta v, dot, w -= alpha v, norm, w / beta),

@@ -1,0 +1,155 @@
+"""GPU parity of the batching row: block-of-vectors kernel, schedule variants,
+autotuner and multi-sector graph batches, each against the oracle (ComplexF64
+within 1e-12, ComplexF32 within 2e-6 relative) and against the eager
+single-vector path (bitwise, where the same kernel runs)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.models import blbq, j1j2, rand_state, xxz
+
+pytestmark = pytest.mark.gpu
+
+OKIND = {"ProjectedBasis": "projected", "TranslationalBasis": "translational",
+         "TranslationParityBasis": "translation_parity"}
+
+# one case per phase mode of the specialised kernel (real positive, real
+# signs, complex), the unit-cell form and the projected basis
+BLOCK_CASES = [
+    ("TranslationParityBasis", 16, dict(N=8, k=0, p=1), xxz),
+    ("TranslationParityBasis", 16, dict(N=8, k=8, p=-1), j1j2),
+    ("TranslationalBasis", 16, dict(N=8, k=0), j1j2),
+    ("TranslationalBasis", 16, dict(N=8, k=8), xxz),
+    ("TranslationalBasis", 16, dict(N=7, k=3), j1j2),
+    ("TranslationalBasis", 16, dict(N=8, k=3, a=2), xxz),
+    ("ProjectedBasis", 14, dict(N=7), j1j2),
+]
+
+
+def block(D, nb, seed, dtype):
+    return np.stack([rand_state(D, seed + c) for c in range(nb)], axis=1).astype(dtype)
+
+
+@pytest.mark.parametrize("kind,L,kw,model", BLOCK_CASES)
+@pytest.mark.parametrize("nb", [2, 4, 8])
+def test_block_matvec_matches_oracle(gpu, kind, L, kw, model, nb):
+    import edkit_hip as edk
+    B = getattr(edk, kind)(L=L, **kw)
+    ob = orc.Basis(OKIND[kind], L, **kw)
+    H = model(B)
+    op = orc.Op(ob, H.flatten())
+    for dt, tol in ((np.complex128, 1e-12), (np.complex64, 2e-6)):
+        X = block(B.D, nb, 100 * nb, dt)
+        Y = H @ edk.DeviceArray.from_host(X)
+        Yh = Y.to_host()
+        for c in range(nb):
+            yo = op @ X[:, c].astype(np.complex128)
+            assert np.linalg.norm(Yh[:, c] - yo) <= tol * np.linalg.norm(yo), (kind, kw, nb, dt, c)
+        # accumulate: mul!(Y, H, X) adds to Y
+        Y2 = edk.DeviceArray.from_host(X)
+        edk.mul_(Y2, H, edk.DeviceArray.from_host(X))
+        assert np.linalg.norm(Y2.to_host() - (X + Yh)) <= tol * np.linalg.norm(X + Yh)
+
+
+def test_schedule_variants_agree(gpu):
+    import edkit_hip as edk
+    from edkit_hip import _lib
+    from edkit_hip.matvec import device_operator
+    for kind, L, kw, model in BLOCK_CASES:
+        B = getattr(edk, kind)(L=L, **kw)
+        ob = orc.Basis(OKIND[kind], L, **kw)
+        H = model(B)
+        op = device_operator(H)
+        yo = orc.Op(ob, H.flatten()) @ rand_state(B.D, 5)
+        for dt, tol in ((np.complex128, 1e-12), (np.complex64, 2e-6)):
+            x = edk.DeviceArray.from_host(rand_state(B.D, 5, dt))
+            for v in (_lib.EDK_VARIANT_GENERIC, _lib.EDK_VARIANT_TILE, _lib.EDK_VARIANT_LANE2,
+                      _lib.EDK_VARIANT_LANE4, _lib.EDK_VARIANT_LANE8):
+                op.set_variant(dt, 1, v)
+                y = (H @ x).to_host()
+                assert np.linalg.norm(y - yo) <= tol * np.linalg.norm(yo), (kind, kw, dt, v)
+            op.set_variant(dt, 1, _lib.EDK_VARIANT_AUTO)
+    with pytest.raises(edk.EdkError):
+        op.set_variant(np.complex128, 4, _lib.EDK_VARIANT_LANE2)
+
+
+def test_autotune_picks_a_valid_schedule(gpu):
+    import edkit_hip as edk
+    from edkit_hip import _lib
+    from edkit_hip.matvec import device_operator
+    B = edk.TranslationParityBasis(L=20, N=10, k=0, p=1)
+    ob = orc.Basis("translation_parity", 20, N=10, k=0, p=1)
+    H = xxz(B)
+    op = device_operator(H)
+    yo = orc.Op(ob, H.flatten()) @ rand_state(B.D, 3)
+    for nb in (1, 8):
+        v, ms = op.autotune(np.complex128, nb, reps=3)
+        assert v in _lib.VARIANT_NAMES and v != _lib.EDK_VARIANT_AUTO and ms > 0
+    y = (H @ edk.DeviceArray.from_host(rand_state(B.D, 3))).to_host()
+    assert np.linalg.norm(y - yo) <= 1e-12 * np.linalg.norm(yo)
+    # a basis outside the specialised kernel: only the generic schedule exists
+    B3 = edk.TranslationalBasis(L=8, N=8, k=0, base=3)
+    v, _ = device_operator(blbq(B3)).autotune(np.complex64, 1, reps=2)
+    assert v == _lib.EDK_VARIANT_GENERIC
+
+
+@pytest.mark.parametrize("streams", [1, 4])
+def test_multi_sector_graph_batch(gpu, streams):
+    """All 16 momentum sectors of a J1-J2 ring L=16 N=8 (blocks of 8, FP32)
+    plus a spin-1 BLBQ sector, in one graph; every replay equals the eager
+    matvecs bitwise and the oracle to FP32 rounding."""
+    import edkit_hip as edk
+    items, eager, checks = [], [], []
+    for k in range(16):
+        B = edk.TranslationalBasis(L=16, N=8, k=k)
+        H = j1j2(B, 0.3)
+        X = edk.DeviceArray.from_host(block(B.D, 8, 7 * k, np.complex64))
+        Y = edk.DeviceArray((B.D, 8), np.complex64)
+        items.append((H, X, Y))
+        eager.append((H @ X).to_host())
+        if k in (0, 3, 8):
+            checks.append((len(items) - 1, orc.Op(orc.Basis("translational", 16, N=8, k=k), H.flatten())))
+    B3 = edk.TranslationalBasis(L=8, N=8, k=0, base=3)
+    H3 = blbq(B3, 0.2)
+    X3 = edk.DeviceArray.from_host(block(B3.D, 8, 900, np.complex64))
+    Y3 = edk.DeviceArray((B3.D, 8), np.complex64)
+    items.append((H3, X3, Y3))
+    eager.append((H3 @ X3).to_host())
+    checks.append((len(items) - 1, orc.Op(orc.Basis("translational", 8, base=3, N=8, k=0), H3.flatten())))
+    batch = edk.MatvecBatch(items, streams=streams)
+    for rep in range(3):
+        for _, _, Y in items:
+            Y.zero_()
+        batch.launch()
+        gpu.synchronize()
+        for (H, X, Y), ye in zip(items, eager):
+            assert np.array_equal(Y.to_host(), ye), rep
+        # eager work between replays
+        _ = (items[0][0] @ items[0][1]).to_host()
+    for i, op in checks:
+        H, X, Y = items[i]
+        Xh, Yh = X.to_host(), Y.to_host()
+        for c in range(8):
+            yo = op @ Xh[:, c].astype(np.complex128)
+            assert np.linalg.norm(Yh[:, c] - yo) <= 2e-6 * np.linalg.norm(yo)
+    batch.close()
+
+
+def test_batch_accumulates_and_checks_arguments(gpu):
+    import edkit_hip as edk
+    B = edk.TranslationParityBasis(L=14, N=7, k=0, p=1)
+    H = xxz(B)
+    x = edk.DeviceArray.from_host(rand_state(B.D, 1))
+    y0 = rand_state(B.D, 2)
+    y = edk.DeviceArray.from_host(y0)
+    hx = (H @ x).to_host()
+    batch = edk.MatvecBatch([(H, x, y, True)], streams=2)
+    batch.launch()
+    batch.launch()
+    gpu.synchronize()
+    assert np.linalg.norm(y.to_host() - (y0 + 2 * hx)) <= 1e-13 * np.linalg.norm(y0 + 2 * hx)
+    batch.close()
+    with pytest.raises(ValueError):
+        edk.MatvecBatch([(H, x, edk.DeviceArray((B.D,), np.complex64))])
+    with pytest.raises(edk.EdkError):
+        edk.MatvecBatch([(H, x, x)])
