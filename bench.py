@@ -246,6 +246,134 @@ def c4_workload(args, ev):
     print(json.dumps(out))
 
 
+def c5_stream(n, rate, seed):
+    """Seeded serving stream: Poisson arrivals at `rate` req/s; jobs uniform
+    over spin-1/2 L in {18,...,30} (XXZ Delta in [0.5, 2] or J1-J2 J2 in
+    [0, 0.6]) and spin-1 BLBQ L in {12, 14, 16} (beta in [0, 0.3]), Sz = 0,
+    random momentum k (and parity p at k = 0, L/2); 60 % ground state to
+    residual 1e-9, 40 % Krylov evolution to t in [1, 20]; each job starts from
+    the product state of a seeded random representative of its sector."""
+    rng = np.random.default_rng(seed)
+    t, jobs = 0.0, []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        fam = int(rng.integers(10))
+        if fam < 7:
+            L, base = 18 + 2 * fam, 2
+            model = ("xxz", float(rng.uniform(0.5, 2.0))) if rng.random() < 0.5 else \
+                ("j1j2", float(rng.uniform(0.0, 0.6)))
+        else:
+            L, base = 12 + 2 * (fam - 7), 3
+            model = ("blbq", float(rng.uniform(0.0, 0.3)))
+        k = int(rng.integers(L))
+        p = int(rng.choice([1, -1])) if k in (0, L // 2) else 0
+        kind = "ground" if rng.random() < 0.6 else "evolve"
+        jobs.append(dict(arrival=t, L=L, base=base, model=model, k=k, p=p, kind=kind,
+                         t=float(rng.uniform(1.0, 20.0)), start=float(rng.random())))
+    return jobs
+
+
+def c5_workload(args, ev):
+    """Serving stream through the server (continuous admission, `workers`
+    streams) versus the same jobs solved one at a time."""
+    import edkit_hip as edk
+    sys.path.insert(0, ROOT)
+    from tests.models import blbq, j1j2, xxz
+    jobs = c5_stream(args.requests, args.rate, 5)
+    bases = {}
+
+    def basis_of(j):
+        key = (j["L"], j["base"], j["k"], j["p"])
+        if key not in bases:
+            N = j["L"] // 2 if j["base"] == 2 else j["L"]
+            if j["p"]:
+                bases[key] = edk.TranslationParityBasis(L=j["L"], N=N, k=j["k"], p=j["p"], base=j["base"])
+            else:
+                bases[key] = edk.TranslationalBasis(L=j["L"], N=N, k=j["k"], base=j["base"])
+        return bases[key]
+
+    def operator_of(j):
+        B = basis_of(j)
+        name, c = j["model"]
+        return xxz(B, c) if name == "xxz" else (j1j2(B, c) if name == "j1j2" else blbq(B, c))
+
+    def submit(srv, j, H):
+        idx = min(int(j["start"] * H.B.D), H.B.D - 1)
+        if j["kind"] == "ground":
+            return srv.submit_ground(H, psi0_index=idx, m=300, tol=1e-9)
+        return srv.submit_evolve(H, j["t"], psi0_index=idx)
+
+    # served: requests submitted at their arrival times
+    ops = []
+    with edk.Server(workers=args.workers) as srv:
+        t0 = srv.clock()
+        tickets = []
+        for j in jobs:
+            now = srv.clock() - t0
+            if j["arrival"] > now:
+                time.sleep(j["arrival"] - now)
+            H = operator_of(j)
+            ops.append(H)
+            tickets.append(submit(srv, j, H))
+        res = []
+        for i, tk in enumerate(tickets):
+            r, _ = srv.wait(tk)
+            res.append(r)
+            if (i + 1) % 50 == 0:
+                print(f"[c5] served {i + 1}/{len(tickets)}", file=sys.stderr, flush=True)
+        st = srv.stats()
+    bad = [r["message"] for r in res if r["status"] != 0]
+    lat = np.array([r["t_end"] - t0 - j["arrival"] for r, j in zip(res, jobs)])
+    span = max(r["t_end"] for r in res) - t0
+    served = len(jobs) / span
+    # one at a time, same jobs and operators
+    seq_res = []
+    with edk.Server(workers=1) as srv:
+        s0 = srv.clock()
+        for i, (j, H) in enumerate(zip(jobs, ops)):
+            r, _ = srv.wait(submit(srv, j, H))
+            seq_res.append(r)
+            if (i + 1) % 50 == 0:
+                print(f"[c5] one-at-a-time {i + 1}/{len(jobs)}", file=sys.stderr, flush=True)
+        seq_span = srv.clock() - s0
+    seq = len(jobs) / seq_span
+    same = all(a["E0"] == b["E0"] and a["overlap_re"] == b["overlap_re"] and a["overlap_im"] == b["overlap_im"]
+               for a, b in zip(res, seq_res))
+    ground = [r for r, j in zip(res, jobs) if j["kind"] == "ground"]
+    # where the one-at-a-time time goes: per (L, kind) count, mean job seconds,
+    # mean matvecs and share of the total
+    classes = {}
+    for r, j in zip(seq_res, jobs):
+        c = classes.setdefault(f"L{j['L']}{'s1' if j['base'] == 3 else ''}-{j['kind']}", [0, 0.0, 0])
+        c[0] += 1
+        c[1] += r["t_end"] - r["t_start"]
+        c[2] += r["matvecs"]
+    tot = sum(c[1] for c in classes.values())
+    breakdown = {k: {"n": c[0], "mean_s": c[1] / c[0], "mean_matvecs": c[2] / c[0], "share": c[1] / tot}
+                 for k, c in sorted(classes.items(), key=lambda kv: -kv[1][1])}
+    out = {
+        "metric": "mixed-solve req/s (served) with p50/p99 latency",
+        "value": served, "unit": "req/s", "n_gpus": 1, "steps": len(jobs), "warmup": 0,
+        "ms_per_step": 1e3 / served, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128", "data": "synthetic (seeded Poisson stream, product-state starts)",
+        "config": {"workload": f"serving stream: {len(jobs)} requests, Poisson {args.rate} req/s, spin-1/2 "
+                               "L=18..30 XXZ/J1-J2 and spin-1 BLBQ L=12..16, random k/p sectors, Sz=0; "
+                               "60% Lanczos to residual 1e-9, 40% Krylov evolution to t in [1,20]",
+                   "workers": args.workers, "parallelism": "single GPU"},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean())},
+        "one_at_a_time_req_s": seq,
+        "speedup_vs_one_at_a_time": served / seq,
+        "results_identical_alone_and_served": bool(same),
+        "failed": len(bad), "first_error": bad[0] if bad else None,
+        "max_ground_resid": max((r["resid"] for r in ground), default=None),
+        "peak_concurrent_jobs": st["peak_running"], "peak_inflight_GB": st["peak_inflight_bytes"] / 1e9,
+        "sectors_built": len(bases),
+        "one_at_a_time_breakdown": breakdown,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,8 +383,12 @@ def main():
     ap.add_argument("--c32", action="store_true", help="also time the ComplexF32 form")
     ap.add_argument("--copy-probe", action="store_true",
                     help="(kept for profiling commands; the copy probe always runs)")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c4"],
-                    help="c3: L=32 H*psi roofline line (default); c4: batched uneven sectors")
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c5"],
+                    help="c3: L=32 H*psi roofline line (default); c4: batched uneven sectors; "
+                         "c5: serving stream")
+    ap.add_argument("--requests", type=int, default=2000, help="c5: requests in the stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate, req/s")
+    ap.add_argument("--workers", type=int, default=8, help="c5: server worker streams")
     ap.add_argument("--lanczos-steps", type=int, default=300,
                     help="steps of the ground-state Lanczos leg (0 skips it)")
     args = ap.parse_args()
@@ -269,6 +401,9 @@ def main():
     ev = HipEvents()
     if args.workload == "c4":
         c4_workload(args, ev)
+        return
+    if args.workload == "c5":
+        c5_workload(args, ev)
         return
     t0 = time.perf_counter()
     B = edk.TranslationParityBasis(L=L, N=NUP, k=0, p=1)

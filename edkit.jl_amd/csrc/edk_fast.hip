@@ -1,5 +1,7 @@
 // edk_fast.hip -- specialised gather kernels for one vector, and the host
 // setup they share with the block kernels (edk_fast.h describes the method).
+#include <mutex>
+
 #include "edk_fast.h"
 
 template <int KIND, int PH, int NB, int LT, typename VT, typename ST>
@@ -398,7 +400,10 @@ static bool find_families(const edk_op *op, FastTab &tab) {
   return true;
 }
 
+static std::mutex g_setup_mu;  // first-time setup may be reached from several threads
+
 static bool fast_setup(edk_op *op) {
+  std::lock_guard<std::mutex> setup_lock(g_setup_mu);
   FastState *S = fast_state(op);
   if (S->checked) return S->ok;
   S->checked = true;

@@ -38,7 +38,19 @@ struct edk_ctx {
   hipStream_t stream;
   bool own_stream;
   int num_cus;
+  // stream-ordered allocation (hipMallocAsync/hipFreeAsync on `stream`) for
+  // the Krylov drivers' per-call buffers: a plain hipFree waits for the whole
+  // device, which would serialise contexts that run side by side
+  bool async_alloc = false;
+  // two pinned slots of EDK_PINNED_SLOT double2 for reduction results, owned
+  // by the context (no pinned allocation per call)
+  void *pinned = nullptr;
 };
+#define EDK_PINNED_SLOT 1040
+
+// device buffers of a context's calls (see edk_ctx::async_alloc)
+hipError_t edk_dalloc(edk_ctx *c, void **p, size_t bytes);
+void edk_dfree(edk_ctx *c, void *p);
 
 // Generator kinds of the Abelian canonicaliser.
 enum { GEN_ROTL = 0, GEN_REV = 1, GEN_IDENT = 2, GEN_BENES = 3 };

@@ -14,7 +14,8 @@ struct KrylovWork {
   double2 *d_part = nullptr;  // block partials
   double2 *d_s = nullptr;     // reduced coefficients
   double2 *h_s = nullptr;     // pinned host mirror
-  int init(edk_ctx *c, int64_t n, int dtype, int maxm);
+  // slot: which of the context's two pinned scalar buffers to use
+  int init(edk_ctx *c, int64_t n, int dtype, int maxm, int slot = 0);
   void release();
   size_t esize() const;
   int dot(const void *a, const void *b, double2 *out);  // sum conj(a) b (b == null: |a|^2)
@@ -29,3 +30,10 @@ int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, d
                     const int *rows);
 int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta, int *m,
                    int m_target, int64_t *matvecs, int ncols);
+
+// the Lanczos ground state and the Krylov cache on a given context (its stream
+// and allocation mode), for callers that run several at once
+int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, double tol, int refine, double *E0,
+                    double *resid, int *steps, void *vec_out);
+int edk_krylov_create_on(edk_op *op, edk_ctx *ctx, int dtype, const void *psi0, const edk_krylov_params *p,
+                         edk_krylov **out);

@@ -190,24 +190,25 @@ static int red_blocks(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
 }
 
-int KrylovWork::init(edk_ctx *c, int64_t n_, int dtype_, int maxm) {
+int KrylovWork::init(edk_ctx *c, int64_t n_, int dtype_, int maxm, int slot) {
   ctx = c;
   n = n_;
   dtype = dtype_;
   nblk = red_blocks(n);
+  if (maxm + 2 > EDK_PINNED_SLOT) return edk_fail(EDK_ERR_UNSUPPORTED, "Krylov dimension above 1024");
   size_t pb = (size_t)nblk * (maxm + 2) * sizeof(double2);
-  if (hipMalloc((void **)&d_part, pb) != hipSuccess) return edk_fail(EDK_ERR_OOM, "krylov partials");
-  if (hipMalloc((void **)&d_s, (maxm + 2) * sizeof(double2)) != hipSuccess)
+  if (edk_dalloc(ctx, (void **)&d_part, pb) != hipSuccess) return edk_fail(EDK_ERR_OOM, "krylov partials");
+  if (edk_dalloc(ctx, (void **)&d_s, (maxm + 2) * sizeof(double2)) != hipSuccess)
     return edk_fail(EDK_ERR_OOM, "krylov coefficients");
-  if (hipHostMalloc((void **)&h_s, (maxm + 2) * sizeof(double2)) != hipSuccess)
-    return edk_fail(EDK_ERR_OOM, "krylov host buffer");
+  h_s = (double2 *)ctx->pinned + (size_t)slot * EDK_PINNED_SLOT;
   return EDK_OK;
 }
 
 void KrylovWork::release() {
-  hipFree(d_part);
-  hipFree(d_s);
-  if (h_s) hipHostFree(h_s);
+  if (ctx) {
+    edk_dfree(ctx, d_part);
+    edk_dfree(ctx, d_s);
+  }
   d_part = nullptr;
   d_s = nullptr;
   h_s = nullptr;
@@ -370,7 +371,7 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
   size_t es = K.esize();
   for (int j = *m + 1; j <= m_target; j++) {
     char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
-    if (int rc = edk_matvec_launch(op, K.dtype, w, vj, 1, 0)) return -rc;
+    if (int rc = edk_matvec_launch(op, K.dtype, w, vj, 1, 0, K.ctx->stream)) return -rc;
     (*matvecs)++;
     if (j > 1) {
       char *vjm1 = (char *)V + (size_t)(j - 2) * ldv * es;
@@ -403,12 +404,11 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
 // ---------------------------------------------------------------------------
 // Lanczos ground state
 // ---------------------------------------------------------------------------
-extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, double tol, int refine, double *E0,
-                                  double *resid, int *steps, void *vec_out) {
-  if (!op || !x0 || !E0 || !resid || m < 1) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground: bad argument");
+int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, double tol, int refine, double *E0,
+                    double *resid, int *steps, void *vec_out) {
+  if (!op || !ctx || !x0 || !E0 || !resid || m < 1) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground: bad argument");
   if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "bad dtype");
   if (m > EDK_KRYLOV_MAXM) return edk_fail(EDK_ERR_UNSUPPORTED, "Lanczos dimension above 1024");
-  edk_ctx *ctx = op->ctx;
   int64_t D = op->basis->D;
   if (D < 1) return edk_fail(EDK_ERR_ARG, "empty sector");
   if (m > D) m = (int)D;
@@ -420,8 +420,8 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
   }
   size_t es = K.esize();
   void *V = nullptr, *w = nullptr;
-  if (hipMalloc(&V, (size_t)D * (m + 1) * es) != hipSuccess || hipMalloc(&w, (size_t)D * es) != hipSuccess) {
-    hipFree(V);
+  if (edk_dalloc(ctx, &V, (size_t)D * (m + 1) * es) != hipSuccess || edk_dalloc(ctx, &w, (size_t)D * es) != hipSuccess) {
+    edk_dfree(ctx, V);
     K.release();
     return edk_fail(EDK_ERR_OOM, "Lanczos basis does not fit in device memory");
   }
@@ -453,14 +453,14 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
     for (int k = 0; k < mm; k++) c[k] = make_double2(Q[k], 0.0);
     // Ritz vector in FP64 storage (accumulated in FP64 from the basis)
     void *y = nullptr, *hy = nullptr;
-    if (hipMalloc(&y, (size_t)D * 16) != hipSuccess) rc = edk_fail(EDK_ERR_OOM, "Ritz vector");
+    if (edk_dalloc(ctx, &y, (size_t)D * 16) != hipSuccess) rc = edk_fail(EDK_ERR_OOM, "Ritz vector");
     if (!rc) rc = K.combine(V, D, mm, c.data(), 1.0, y, EDK_C64);
     if (!rc && refine) {
       // FP64 Rayleigh quotient and residual with the ComplexF64 matvec
       KrylovWork K64;
-      if (hipMalloc(&hy, (size_t)D * 16) != hipSuccess) rc = edk_fail(EDK_ERR_OOM, "refinement vector");
-      if (!rc) rc = K64.init(ctx, D, EDK_C64, 1);
-      if (!rc) rc = edk_matvec_launch(op, EDK_C64, hy, y, 1, 0);
+      if (edk_dalloc(ctx, &hy, (size_t)D * 16) != hipSuccess) rc = edk_fail(EDK_ERR_OOM, "refinement vector");
+      if (!rc) rc = K64.init(ctx, D, EDK_C64, 1, 1);
+      if (!rc) rc = edk_matvec_launch(op, EDK_C64, hy, y, 1, 0, ctx->stream);
       double2 num, den;
       if (!rc) rc = K64.dot(y, hy, &num);
       if (!rc) rc = K64.dot(y, y, &den);
@@ -475,7 +475,7 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
         }
       }
       K64.release();
-      hipFree(hy);
+      edk_dfree(ctx, hy);
     }
     if (!rc && vec_out) {
       if (dtype == EDK_C64) rc = hipMemcpyAsync(vec_out, y, (size_t)D * 16, hipMemcpyDeviceToDevice, ctx->stream) == hipSuccess ? EDK_OK : edk_fail(EDK_ERR_HIP, "copy Ritz vector");
@@ -484,12 +484,13 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
       }
       if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = edk_fail(EDK_ERR_HIP, "Ritz vector sync");
     }
-    hipFree(y);
+    edk_dfree(ctx, y);
   }
   hipStreamSynchronize(ctx->stream);
-  hipFree(V);
-  hipFree(w);
+  edk_dfree(ctx, V);
+  edk_dfree(ctx, w);
   K.release();
+  if (ctx->async_alloc) hipStreamSynchronize(ctx->stream);
   if (rc) return rc;
   *E0 = e0;
   *resid = res;
@@ -497,11 +498,18 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
   return EDK_OK;
 }
 
+extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, double tol, int refine, double *E0,
+                                  double *resid, int *steps, void *vec_out) {
+  if (!op) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground: null operator");
+  return edk_lanczos_run(op, op->ctx, dtype, x0, m, tol, refine, E0, resid, steps, vec_out);
+}
+
 // ---------------------------------------------------------------------------
 // KrylovEvolutionCache
 // ---------------------------------------------------------------------------
 struct edk_krylov {
   edk_op *H;
+  edk_ctx *ctx;  // context (stream) the cache runs on
   int dtype;
   int64_t N;
   double t_anchor = 0, t_latest = 0;
@@ -693,16 +701,17 @@ static int ensure_valid(edk_krylov *c, double target) {
 
 static void krylov_free(edk_krylov *c) {
   if (!c) return;
-  hipFree(c->psi_anchor);
-  hipFree(c->V);
-  hipFree(c->w);
+  edk_dfree(c->ctx, c->psi_anchor);
+  edk_dfree(c->ctx, c->V);
+  edk_dfree(c->ctx, c->w);
   c->K.release();
+  if (c->ctx->async_alloc) (void)hipStreamSynchronize(c->ctx->stream);
   delete c;
 }
 
-extern "C" int edk_krylov_create(edk_op *op, int dtype, const void *psi0, const edk_krylov_params *p,
-                                 edk_krylov **out) {
-  if (!op || !psi0 || !out) return edk_fail(EDK_ERR_ARG, "edk_krylov_create: null argument");
+int edk_krylov_create_on(edk_op *op, edk_ctx *ctx, int dtype, const void *psi0, const edk_krylov_params *p,
+                         edk_krylov **out) {
+  if (!op || !ctx || !psi0 || !out) return edk_fail(EDK_ERR_ARG, "edk_krylov_create: null argument");
   if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "bad dtype");
   edk_krylov_params prm;
   edk_krylov_default_params(&prm);
@@ -718,10 +727,10 @@ extern "C" int edk_krylov_create(edk_op *op, int dtype, const void *psi0, const 
   if (prm.bisect_iters < 0) return edk_fail(EDK_ERR_ARG, "bisect_iters must be >= 0");
   if (prm.extend_step < 1) return edk_fail(EDK_ERR_ARG, "extend_step must be >= 1");
   if (prm.m_max > EDK_KRYLOV_MAXM) return edk_fail(EDK_ERR_UNSUPPORTED, "m_max above 1024");
-  edk_ctx *ctx = op->ctx;
   EDK_HIP(hipSetDevice(ctx->device));
   edk_krylov *c = new edk_krylov();
   c->H = op;
+  c->ctx = ctx;
   c->dtype = dtype;
   c->N = N;
   c->tol = prm.tol;
@@ -737,8 +746,9 @@ extern "C" int edk_krylov_create(edk_op *op, int dtype, const void *psi0, const 
   c->beta.assign(c->m_max, 0.0);
   int rc = c->K.init(ctx, N, dtype, c->m_max + 1);
   size_t es = c->K.esize();
-  if (!rc && (hipMalloc(&c->psi_anchor, N * es) != hipSuccess || hipMalloc(&c->V, (size_t)N * (c->m_max + 1) * es) != hipSuccess ||
-              hipMalloc(&c->w, N * es) != hipSuccess))
+  if (!rc && (edk_dalloc(ctx, &c->psi_anchor, N * es) != hipSuccess ||
+              edk_dalloc(ctx, &c->V, (size_t)N * (c->m_max + 1) * es) != hipSuccess ||
+              edk_dalloc(ctx, &c->w, N * es) != hipSuccess))
     rc = edk_fail(EDK_ERR_OOM, "Krylov cache does not fit in device memory");
   if (!rc && hipMemcpyAsync(c->psi_anchor, psi0, N * es, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
     rc = edk_fail(EDK_ERR_HIP, "copy psi0");
@@ -761,9 +771,15 @@ extern "C" int edk_krylov_create(edk_op *op, int dtype, const void *psi0, const 
   return EDK_OK;
 }
 
+extern "C" int edk_krylov_create(edk_op *op, int dtype, const void *psi0, const edk_krylov_params *p,
+                                 edk_krylov **out) {
+  if (!op) return edk_fail(EDK_ERR_ARG, "edk_krylov_create: null argument");
+  return edk_krylov_create_on(op, op->ctx, dtype, psi0, p, out);
+}
+
 extern "C" int edk_krylov_evolve(edk_krylov *c, double t, void *out) {
   if (!c || !out) return edk_fail(EDK_ERR_ARG, "edk_krylov_evolve: null argument");
-  EDK_HIP(hipSetDevice(c->H->ctx->device));
+  EDK_HIP(hipSetDevice(c->ctx->device));
   if (t < c->t_latest)
     return edk_fail(EDK_ERR_ARG, "timeevolve!: requested time precedes the last served time; backward evolution is not supported");
   double tau = t - c->t_anchor;
@@ -772,7 +788,7 @@ extern "C" int edk_krylov_evolve(edk_krylov *c, double t, void *out) {
   if (int rc = reconstruct(c, t - c->t_anchor, out)) return rc;
   c->t_latest = t;
   c->diag.total_times_served++;
-  EDK_HIP(hipStreamSynchronize(c->H->ctx->stream));
+  EDK_HIP(hipStreamSynchronize(c->ctx->stream));
   return EDK_OK;
 }
 
@@ -791,8 +807,8 @@ extern "C" int edk_krylov_accepted_intervals(edk_krylov *c, double *out, int64_t
 
 extern "C" int edk_krylov_destroy(edk_krylov *c) {
   if (!c) return EDK_OK;
-  hipSetDevice(c->H->ctx->device);
-  hipStreamSynchronize(c->H->ctx->stream);
+  hipSetDevice(c->ctx->device);
+  hipStreamSynchronize(c->ctx->stream);
   krylov_free(c);
   return EDK_OK;
 }

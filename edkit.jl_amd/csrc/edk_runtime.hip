@@ -37,8 +37,26 @@ extern "C" int edk_ctx_create(int device, void *stream, edk_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
   else c->num_cus = 256;
+  hipError_t e = hipHostMalloc(&c->pinned, 2 * EDK_PINNED_SLOT * sizeof(double2));
+  if (e != hipSuccess) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return edk_fail(EDK_ERR_OOM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
   *out = c;
   return EDK_OK;
+}
+
+hipError_t edk_dalloc(edk_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (c->async_alloc) return hipMallocAsync(p, bytes, c->stream);
+  return hipMalloc(p, bytes);
+}
+
+void edk_dfree(edk_ctx *c, void *p) {
+  if (!p) return;
+  if (c->async_alloc) (void)hipFreeAsync(p, c->stream);
+  else (void)hipFree(p);
 }
 
 extern "C" int edk_ctx_destroy(edk_ctx *c) {
@@ -46,6 +64,7 @@ extern "C" int edk_ctx_destroy(edk_ctx *c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   if (c->own_stream) hipStreamDestroy(c->stream);
+  if (c->pinned) hipHostFree(c->pinned);
   delete c;
   return EDK_OK;
 }

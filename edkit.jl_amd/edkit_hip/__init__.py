@@ -15,4 +15,5 @@ from .device import Context, DeviceArray, default_context, set_default_context  
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
 from .matvec import apply, device_operator, matvec_into, mul, mul_  # noqa: F401
 from .operator import Operator, find_base, operator, trans_inv_operator  # noqa: F401
+from .serve import Server  # noqa: F401
 from .spin import spin  # noqa: F401

@@ -28,6 +28,8 @@ SYMBOLS = [
     "edk_matvec", "edk_matvec_host",
     "edk_operator_set_variant", "edk_operator_autotune",
     "edk_batch_create", "edk_batch_launch", "edk_batch_destroy",
+    "edk_server_create", "edk_server_submit", "edk_server_wait", "edk_server_clock",
+    "edk_server_stats", "edk_server_destroy",
     "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2",
     "edk_lanczos_ground",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
@@ -54,6 +56,29 @@ class BasisSpec(C.Structure):
 class BatchItem(C.Structure):
     _fields_ = [("op", C.c_void_p), ("x", C.c_void_p), ("y", C.c_void_p), ("nb", C.c_int64),
                 ("accumulate", C.c_int32)]
+
+
+EDK_JOB_GROUND, EDK_JOB_EVOLVE = 0, 1
+
+
+class Job(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("op", C.c_void_p), ("psi0", C.c_void_p),
+                ("psi0_index", C.c_int64), ("m", C.c_int32), ("refine", C.c_int32), ("tol", C.c_double),
+                ("t", C.c_double), ("m_init", C.c_int32), ("m_max", C.c_int32),
+                ("want_vector", C.c_int32)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("steps", C.c_int32), ("E0", C.c_double), ("resid", C.c_double),
+                ("matvecs", C.c_int64), ("overlap_re", C.c_double), ("overlap_im", C.c_double),
+                ("norm", C.c_double), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double), ("message", C.c_char * 160)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("queued", C.c_int64), ("running", C.c_int64),
+                ("peak_running", C.c_int64), ("inflight_bytes", C.c_int64),
+                ("peak_inflight_bytes", C.c_int64), ("budget_bytes", C.c_int64)]
 
 
 class KrylovParams(C.Structure):
@@ -113,6 +138,12 @@ def lib():
         "edk_batch_create": (I32, [P, I32, C.POINTER(BatchItem), I64, I32, PP]),
         "edk_batch_launch": (I32, [P]),
         "edk_batch_destroy": (I32, [P]),
+        "edk_server_create": (I32, [I32, I32, SZ, PP]),
+        "edk_server_submit": (I32, [P, C.POINTER(Job), C.POINTER(C.c_int64)]),
+        "edk_server_wait": (I32, [P, I64, D, C.POINTER(JobResult), P]),
+        "edk_server_clock": (D, [P]),
+        "edk_server_stats": (I32, [P, C.POINTER(ServerInfo)]),
+        "edk_server_destroy": (I32, [P]),
         "edk_vec_scale": (I32, [P, I32, P, P, D, I64]),
         "edk_vec_axpy": (I32, [P, I32, P, P, D, D, I64]),
         "edk_vec_dot": (I32, [P, I32, P, P, I64, C.POINTER(D), C.POINTER(D)]),

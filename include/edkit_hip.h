@@ -243,6 +243,56 @@ int edk_krylov_accepted_intervals(edk_krylov *c, double *out, int64_t cap,
                                   int64_t *n);
 int edk_krylov_destroy(edk_krylov *c);
 
+/* ---- serving --------------------------------------------------------------- */
+/* No counterpart in the reference.  A server runs Lanczos ground-state and
+ * Krylov time-evolution jobs (any sector and operator on its device) on
+ * `workers` threads, each with its own stream, so jobs overlap on the GPU.
+ * Jobs are admitted first-in first-out as a worker frees up and the job's
+ * device footprint fits beside the running ones within hbm_budget bytes
+ * (0: 80 % of the free memory at creation).  The start vector is copied at
+ * submit; the job's operator and basis must stay alive until it finishes.
+ * A job's result is independent of what else runs (fixed-order reductions). */
+#define EDK_JOB_GROUND 0 /* edk_lanczos_ground on the start vector */
+#define EDK_JOB_EVOLVE 1 /* KrylovEvolutionCache(H, psi0) then timeevolve!(cache, t) */
+typedef struct {
+  int32_t kind;        /* EDK_JOB_GROUND or EDK_JOB_EVOLVE */
+  int32_t dtype;       /* EDK_C64 or EDK_C32 */
+  edk_op *op;
+  const void *psi0;    /* HOST start vector (D elements of dtype); NULL: the unit
+                          vector at psi0_index, i.e. productstate (ToolKit.jl:103-109) */
+  int64_t psi0_index;  /* 0-based position for the product state */
+  int32_t m;           /* ground: Lanczos steps at most */
+  int32_t refine;      /* ground: FP64 refinement of the Ritz pair */
+  double tol;          /* ground: stop at |beta_m Q[m,1]| <= tol; evolve: Krylov tol (<= 0: 1e-10) */
+  double t;            /* evolve: final time */
+  int32_t m_init;      /* evolve: Krylov dimensions (<= 0: 30 and 60) */
+  int32_t m_max;
+  int32_t want_vector; /* 1: keep the result vector (ground state or psi(t)) */
+} edk_job;
+typedef struct {
+  int32_t status;      /* EDK_OK or the error code of the job */
+  int32_t steps;       /* ground: Lanczos steps; evolve: restarts */
+  double E0, resid;    /* ground */
+  int64_t matvecs;
+  double overlap_re, overlap_im, norm; /* evolve: <psi0|psi(t)>, ||psi(t)|| */
+  double t_submit, t_start, t_end;     /* seconds on the server's monotonic clock */
+  char message[160];   /* error text when status != EDK_OK */
+} edk_job_result;
+typedef struct {
+  int64_t completed, queued, running, peak_running;
+  int64_t inflight_bytes, peak_inflight_bytes, budget_bytes;
+} edk_server_info;
+typedef struct edk_server edk_server;
+int edk_server_create(int device, int workers, size_t hbm_budget, edk_server **out);
+int edk_server_submit(edk_server *s, const edk_job *job, int64_t *ticket);
+/* Wait for a job (timeout_s < 0: no limit; EDK_ERR_STATE on timeout) and
+ * release its ticket; vec_host (D elements) receives the kept vector. */
+int edk_server_wait(edk_server *s, int64_t ticket, double timeout_s, edk_job_result *res,
+                    void *vec_host);
+double edk_server_clock(edk_server *s); /* the server's clock, seconds */
+int edk_server_stats(edk_server *s, edk_server_info *st);
+int edk_server_destroy(edk_server *s); /* queued jobs are dropped, running ones finish */
+
 #ifdef __cplusplus
 }
 #endif

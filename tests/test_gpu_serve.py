@@ -1,0 +1,129 @@
+"""GPU tests of the serving loop: every job's result is the same alone (one
+worker, one job at a time), in a mix (several workers, all jobs queued at
+once) and through the direct entry points -- bit for bit, since every
+reduction has a fixed order -- and matches dense diagonalisation / the oracle
+where the sector is small; admission by HBM footprint and error reporting."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.models import blbq, j1j2, rand_state, xxz
+
+pytestmark = pytest.mark.gpu
+
+
+def job_mix(edk):
+    """A mix of sectors (real and complex phases, spin-1/2 and spin-1, fast and
+    generic kernels), ground-state and evolution jobs."""
+    jobs = []
+    B1 = edk.TranslationParityBasis(L=14, N=7, k=0, p=1)
+    jobs.append(("ground", xxz(B1, 1.3), dict(psi0=rand_state(B1.D, 11), m=200, tol=1e-9)))
+    B2 = edk.TranslationalBasis(L=16, N=8, k=3)
+    jobs.append(("ground", j1j2(B2, 0.4), dict(psi0=rand_state(B2.D, 12), m=200, tol=1e-9)))
+    B3 = edk.TranslationalBasis(L=8, N=8, k=0, base=3)
+    jobs.append(("ground", blbq(B3, 0.2), dict(psi0=rand_state(B3.D, 13), m=150, tol=1e-9, refine=True)))
+    B4 = edk.TranslationalBasis(L=16, N=8, k=0)
+    neel = edk.productstate([0, 1] * 8, B4)
+    jobs.append(("evolve", xxz(B4, 0.7), dict(psi0_index=int(np.argmax(neel)), t=2.0)))
+    B5 = edk.TranslationParityBasis(L=14, N=7, k=7, p=-1)
+    jobs.append(("evolve", j1j2(B5, 0.2), dict(psi0=rand_state(B5.D, 15), t=1.5)))
+    B6 = edk.TranslationalBasis(L=18, N=9, k=5)
+    jobs.append(("ground", xxz(B6, 0.9), dict(psi0=rand_state(B6.D, 16), m=120, tol=1e-9)))
+    jobs.append(("evolve", j1j2(B6, 0.5), dict(psi0=rand_state(B6.D, 17), t=1.0, dtype=np.complex64)))
+    return jobs
+
+
+def submit(srv, kind, H, kw):
+    if kind == "ground":
+        return srv.submit_ground(H, want_vector=True, **kw)
+    return srv.submit_evolve(H, want_vector=True, **kw)
+
+
+def test_results_same_alone_and_in_mix(gpu):
+    import edkit_hip as edk
+    jobs = job_mix(edk)
+    alone = []
+    with edk.Server(workers=1) as srv:
+        for kind, H, kw in jobs:
+            t = submit(srv, kind, H, kw)
+            alone.append(srv.wait(t, timeout=120))
+    with edk.Server(workers=4) as srv:
+        tickets = [submit(srv, kind, H, kw) for kind, H, kw in jobs]
+        mixed = [srv.wait(t, timeout=120) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(jobs)
+    for (kind, H, kw), (ra, va), (rm, vm) in zip(jobs, alone, mixed):
+        assert ra["status"] == 0 and rm["status"] == 0, (ra["message"], rm["message"])
+        assert np.array_equal(va, vm), kind
+        for f in ("E0", "resid", "steps", "matvecs", "overlap_re", "overlap_im", "norm"):
+            assert ra[f] == rm[f], (kind, f)
+    # the direct entry points give the same numbers
+    for (kind, H, kw), (ra, va) in zip(jobs, alone):
+        if kind != "ground":
+            continue
+        E, r, steps, vec = edk.lanczos_ground(H, kw["psi0"], m=kw["m"], tol=kw["tol"],
+                                              refine=kw.get("refine", False), want_vec=True)
+        assert (E, r, steps) == (ra["E0"], ra["resid"], ra["steps"])
+        assert np.array_equal(vec.to_host(), va)
+
+
+def test_served_results_match_dense(gpu):
+    """Ground energies against dense diagonalisation of the oracle's sector
+    matrix, and evolved states against a dense exponential."""
+    import edkit_hip as edk
+    import scipy.linalg as sla
+    B = edk.TranslationParityBasis(L=14, N=7, k=0, p=1)
+    ob = orc.Basis("translation_parity", 14, N=7, k=0, p=1)
+    H = xxz(B, 1.3)
+    op = orc.Op(ob, H.flatten())
+    Hd = np.stack([op @ np.eye(B.D)[:, j] for j in range(B.D)], axis=1)
+    w, _ = np.linalg.eigh(0.5 * (Hd + Hd.conj().T))
+    psi = rand_state(B.D, 3)
+    with edk.Server(workers=2) as srv:
+        tg = srv.submit_ground(H, psi0=psi, m=200, tol=1e-10)
+        te = srv.submit_evolve(H, 1.7, psi0=psi, want_vector=True)
+        rg, _ = srv.wait(tg, timeout=120)
+        re_, v = srv.wait(te, timeout=120)
+    assert abs(rg["E0"] - w[0]) <= 1e-9 * max(1.0, abs(w[0]))
+    ref = sla.expm(-1j * 1.7 * Hd) @ psi
+    assert np.linalg.norm(v - ref) <= 1e-8
+    assert abs(re_["norm"] - 1.0) <= 1e-9
+
+
+def test_admission_and_concurrency(gpu):
+    import edkit_hip as edk
+    B = edk.TranslationParityBasis(L=20, N=10, k=0, p=1)
+    Hs = [xxz(B, 0.5 + 0.1 * i) for i in range(6)]
+    psi = rand_state(B.D, 4)
+    # a budget of one byte: every job runs alone, all still complete
+    with edk.Server(workers=4, hbm_budget=1) as srv:
+        ts = [srv.submit_ground(H, psi0=psi, m=60, tol=0.0) for H in Hs]
+        res = [srv.wait(t, timeout=120)[0] for t in ts]
+        st = srv.stats()
+    assert all(r["status"] == 0 for r in res)
+    assert st["peak_running"] == 1
+    # ample budget: jobs overlap
+    with edk.Server(workers=4) as srv:
+        ts = [srv.submit_ground(H, psi0=psi, m=60, tol=0.0) for H in Hs]
+        res2 = [srv.wait(t, timeout=120)[0] for t in ts]
+        st2 = srv.stats()
+    assert st2["peak_running"] >= 2
+    for a, b in zip(res, res2):
+        assert a["E0"] == b["E0"] and a["resid"] == b["resid"]
+
+
+def test_server_errors(gpu):
+    import edkit_hip as edk
+    B = edk.TranslationParityBasis(L=12, N=6, k=0, p=1)
+    H = xxz(B)
+    with edk.Server(workers=2) as srv:
+        with pytest.raises(edk.EdkError):
+            srv.submit_ground(H, psi0_index=B.D)          # product state outside the sector
+        with pytest.raises(edk.EdkError):
+            srv.submit_ground(H, psi0=rand_state(B.D, 1), m=5000)
+        t = srv.submit_evolve(H, 1.0, psi0=np.zeros(B.D))  # zero start vector: the job fails
+        r, _ = srv.wait(t, timeout=60)
+        assert r["status"] != 0 and "zero norm" in r["message"]
+        t = srv.submit_ground(H, psi0=rand_state(B.D, 2), m=50, tol=1e-9)
+        r, _ = srv.wait(t, timeout=60)
+        assert r["status"] == 0
