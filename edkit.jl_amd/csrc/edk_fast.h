@@ -262,6 +262,83 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
   return true;
 }
 
+// Run-length candidates for L = 32 (unit cell 1), phase mode 0.  With site 1
+// in the top bit, the smallest rotation of a word starts at the beginning of
+// one of its longest zero runs, so only those starts need comparing.  Binary
+// lifting of the zero masks (z_2m = z_m & rotl(z_m, m)) gives the longest run
+// length and the mask of its start positions; the reflected word has the same
+// runs read backwards.  Takes up to two candidates (about 92 % of words at half
+// filling); returns false for a word with more, which the caller hands to
+// canon32_brute.  A word with at most two longest runs has period 32 or 16.
+__device__ __forceinline__ bool canon32_runs(uint32_t s, uint32_t &minA, uint32_t &minB, int &period) {
+  const uint32_t z1 = ~s;
+  const uint32_t z2 = z1 & __builtin_amdgcn_alignbit(z1, z1, 31);
+  const uint32_t z4 = z2 & __builtin_amdgcn_alignbit(z2, z2, 30);
+  const uint32_t z8 = z4 & __builtin_amdgcn_alignbit(z4, z4, 28);
+  const uint32_t z16 = z8 & __builtin_amdgcn_alignbit(z8, z8, 24);
+  uint32_t acc = z16 ? z16 : 0xffffffffu;
+  uint32_t len = z16 ? 16u : 0u;
+#define EDK_RUNSTEP(ZM, M)                                                   \
+  {                                                                          \
+    const uint32_t c_ = acc & __builtin_amdgcn_alignbit(ZM, ZM, (32u - len) & 31u); \
+    const bool t_ = c_ != 0;                                                 \
+    acc = t_ ? c_ : acc;                                                     \
+    len = t_ ? len + (M) : len;                                              \
+  }
+  EDK_RUNSTEP(z8, 8u)
+  EDK_RUNSTEP(z4, 4u)
+  EDK_RUNSTEP(z2, 2u)
+  EDK_RUNSTEP(z1, 1u)
+#undef EDK_RUNSTEP
+  if (__popc(acc) > 2) return false;
+  const uint32_t rb = __brev(s);
+  const uint32_t p1 = __builtin_ctz(acc);  // acc != 0: it only takes non-zero values
+  const uint32_t m2 = acc & (acc - 1);
+  const uint32_t p2 = m2 ? __builtin_ctz(m2) : p1;
+  const uint32_t va1 = __builtin_amdgcn_alignbit(s, s, (p1 + 1) & 31u);
+  const uint32_t va2 = __builtin_amdgcn_alignbit(s, s, (p2 + 1) & 31u);
+  const uint32_t vb1 = __builtin_amdgcn_alignbit(rb, rb, (31u - p1 + len) & 31u);
+  const uint32_t vb2 = __builtin_amdgcn_alignbit(rb, rb, (31u - p2 + len) & 31u);
+  minA = min(va1, va2);
+  minB = min(vb1, vb2);
+  period = (__builtin_amdgcn_alignbit(s, s, 16) == s) ? 16 : 32;
+  return true;
+}
+
+// Full search for phase mode 0 at L = 32: representative and the index of its
+// norm in the staged norm table (period - 1, plus 32 when the reflection
+// maps the orbit onto itself for TranslationParityBasis).
+template <int KIND>
+__device__ __forceinline__ void canon32_brute(uint32_t s, uint32_t &rep, uint32_t &code) {
+  uint32_t minA = s, cur = s;
+#pragma unroll
+  for (int k = 0; k < 15; k++) {
+    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
+    minA = min(minA, min(c1, c2));
+    cur = c2;
+  }
+  minA = min(minA, __builtin_amdgcn_alignbit(cur, cur, 1));
+  int period = 32;
+#pragma unroll
+  for (int k = 16; k >= 1; k >>= 1) period = (__builtin_amdgcn_alignbit(s, s, k) == s) ? k : period;
+  if (KIND == EDK_TRANSLATIONAL) {
+    rep = minA;
+    code = period - 1;
+    return;
+  }
+  uint32_t rb = __brev(s), minB = rb;
+  cur = rb;
+#pragma unroll
+  for (int k = 0; k < 15; k++) {
+    uint32_t c1 = __builtin_amdgcn_alignbit(cur, cur, 1), c2 = __builtin_amdgcn_alignbit(cur, cur, 2);
+    minB = min(minB, min(c1, c2));
+    cur = c2;
+  }
+  minB = min(minB, __builtin_amdgcn_alignbit(cur, cur, 1));
+  rep = min(minA, minB);
+  code = (minA == minB ? 32 : 0) + period - 1;
+}
+
 #define HASH_EMPTY 0xffffffffu
 __device__ __forceinline__ uint32_t hash_bucket(uint32_t key, uint32_t nb) {
   return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
@@ -330,18 +407,18 @@ __device__ __forceinline__ void region_bounds(const FastArgs &A, uint32_t key, u
   hi = rg.x + A.fine[foff + fb + 1];
 }
 
+typedef uint32_t edk_u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint32_t region_find(const FastArgs &A, uint32_t key, uint32_t lo, uint32_t hi) {
   if (hi <= lo) return HASH_EMPTY;
   if (hi - lo <= FINE_SCAN) {
-    uint32_t below = 0;
-    bool eq = false;
-#pragma unroll
-    for (int k = 0; k < FINE_SCAN; k++) {
-      uint32_t v = A.keys[lo + k];  // padded table: reads past the bucket are harmless
-      bool in = lo + k < hi;
-      below += (in && v < key);
-      eq = eq || (in && v == key);
-    }
+    // the bucket's keys in one 16-byte load (dword aligned; the table is
+    // padded, so reading past the bucket or the end is harmless)
+    static_assert(FINE_SCAN == 4, "one 16-byte load covers the scan");
+    edk_u32x4 kv;
+    __builtin_memcpy(&kv, A.keys + lo, sizeof(kv));
+    const uint32_t n = hi - lo;
+    const uint32_t below = (kv.x < key) + (n > 1 && kv.y < key) + (n > 2 && kv.z < key) + (n > 3 && kv.w < key);
+    const bool eq = kv.x == key || (n > 1 && kv.y == key) || (n > 2 && kv.z == key) || (n > 3 && kv.w == key);
     return eq ? lo + below : HASH_EMPTY;
   }
   while (lo < hi) {

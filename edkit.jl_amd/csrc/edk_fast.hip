@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
 }
 
 template <int KIND, int PH, int LT, typename VT, typename ST>
-__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
+__global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_PROJECTED) ? 7 : 1) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate) {
   constexpr bool CPLX = (PH == 2);
   __shared__ double sR[2 * 64];
@@ -143,10 +143,21 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__
   __shared__ double sV[FAST_MAXFAM * FAST_MAXENT];
   __shared__ uint32_t tgt[BAL_CAP];
   __shared__ uint8_t ent[BAL_CAP];
-  __shared__ ST contr[CPLX ? 2 : 2][BAL_CAP];
+  __shared__ __align__(16) ST contr[2][BAL_CAP];
   __shared__ uint32_t wsum[BAL_TPB / 64];
+  __shared__ uint32_t ndef;
+  // L = 32, phase mode 0: run-length candidates, with the few targets that
+  // have more than two longest zero runs deferred to a block-wide full search.
+  // Results go back to LDS: the representative over the target and the
+  // coefficient into the target's own slot of contr[1] (only its owner reads
+  // it, then overwrites it with the contribution); the deferred queue lives
+  // in contr[0], dead before the gathers write there
+  constexpr bool RUNS = (LT == 32 && PH == 0 && KIND != EDK_PROJECTED);
+  uint16_t *dq = reinterpret_cast<uint16_t *>(&contr[0][0]);
+  static_assert(2 * BAL_CAP <= sizeof(contr[0]), "deferred queue must fit in contr[0]");
   const FastTab &T = *A.tab;
   const int tid = threadIdx.x;
+  if (RUNS && tid == 0) ndef = 0;
   if (KIND != EDK_PROJECTED) {
     int nr = (KIND == EDK_TRANSLATION_PARITY ? 2 : 1) * A.len;
     if (tid < nr) sR[tid] = A.Rtab[tid];
@@ -160,10 +171,15 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__
   B.Rtab = sR;
   if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
   const uint32_t r0 = A.chunk[blockIdx.x], nrow = A.chunk[blockIdx.x + 1] - r0;
-  // phase 1: row threads count and write their targets
+  // phase 1: row threads count and write their targets; the row's own x_i
+  // and 1/R_i are loaded now so their latency hides behind phases 1 and 2
   uint32_t s = 0, n = 0;
   const bool own = (uint32_t)tid < nrow;
+  VT xi{};
+  ST invR = 1;
   if (own) {
+    xi = x[r0 + tid];
+    if (KIND != EDK_PROJECTED) invR = (ST)(1.0 / A.Rrow[r0 + tid]);
     s = A.keys[r0 + tid];
     const uint32_t ns = ~s & A.mask;
     for (int f = 0; f < T.nfam; f++) {
@@ -203,22 +219,61 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__
   uint32_t rep[BAL_K], lo[BAL_K], hi[BAL_K], jj[BAL_K];
   bool ok[BAL_K];
   ST cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
-#pragma unroll
-  for (int q = 0; q < BAL_K; q++) {
-    const uint32_t j = tid + BAL_TPB * q;
-    ok[q] = j < total;
-    uint32_t t = ok[q] ? tgt[j] : 0u;
-    double cr = 0.0, ci = 0.0;
-    rep[q] = 0;
-    if (ok[q]) {
-      if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
-      else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
+  if constexpr (RUNS) {
+    // one target at a time, results straight to LDS: little live state
+#pragma unroll 1
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      if (j < total) {
+        uint32_t a, b;
+        int per;
+        if (canon32_runs(tgt[j], a, b, per)) {
+          const int code = (KIND == EDK_TRANSLATIONAL) ? per - 1 : (a == b ? 32 : 0) + per - 1;
+          tgt[j] = (KIND == EDK_TRANSLATIONAL) ? a : min(a, b);
+          contr[1][j] = (ST)(sV[ent[j]] * sR[code]);
+        } else {
+          dq[atomicAdd(&ndef, 1u)] = (uint16_t)j;
+        }
+      }
     }
-    lo[q] = hi[q] = 0;
-    if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
-    double v = ok[q] ? sV[ent[j]] : 0.0;
-    cfr[q] = (ST)(v * cr);
-    if (CPLX) cfi[q] = (ST)(v * ci);
+    __syncthreads();
+    const uint32_t nd = ndef;
+    for (uint32_t d = tid; d < nd; d += BAL_TPB) {
+      const uint32_t j = dq[d];
+      uint32_t r, c;
+      canon32_brute<KIND>(tgt[j], r, c);
+      tgt[j] = r;
+      contr[1][j] = (ST)(sV[ent[j]] * sR[c]);
+    }
+    // results visible; the queue is dead before the contributions reuse it
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      ok[q] = j < total;
+      rep[q] = ok[q] ? tgt[j] : 0u;
+      cfr[q] = ok[q] ? contr[1][j] : (ST)0;
+      lo[q] = hi[q] = 0;
+      if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      ok[q] = j < total;
+      uint32_t t = ok[q] ? tgt[j] : 0u;
+      double cr = 0.0, ci = 0.0;
+      rep[q] = 0;
+      if (ok[q]) {
+        if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
+        else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
+      }
+      lo[q] = hi[q] = 0;
+      if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
+      double v = ok[q] ? sV[ent[j]] : 0.0;
+      cfr[q] = (ST)(v * cr);
+      if (CPLX) cfi[q] = (ST)(v * ci);
+    }
   }
 #pragma unroll
   for (int q = 0; q < BAL_K; q++) jj[q] = ok[q] ? region_find(A, rep[q], lo[q], hi[q]) : HASH_EMPTY;
@@ -258,8 +313,6 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal(FastArgs A, const VT *__
       orr += contr[0][q];
       oi += contr[1][q];
     }
-    VT xi = x[i];
-    ST invR = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[i]);
     ST yr = dg * (ST)xi.x + orr * invR;
     ST yi = dg * (ST)xi.y + oi * invR;
     VT out;

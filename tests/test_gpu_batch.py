@@ -153,3 +153,25 @@ def test_batch_accumulates_and_checks_arguments(gpu):
         edk.MatvecBatch([(H, x, edk.DeviceArray((B.D,), np.complex64))])
     with pytest.raises(edk.EdkError):
         edk.MatvecBatch([(H, x, x)])
+
+
+@pytest.mark.parametrize("kind,kw", [("TranslationParityBasis", dict(N=16, k=0, p=1)),
+                                     ("TranslationalBasis", dict(N=16, k=0))])
+def test_l32_tile_schedule_matches_generic(gpu, kind, kw):
+    """L = 32 sectors whose coefficients are all positive take the run-length
+    canonicaliser in the tile schedule; the generic kernel searches every
+    image.  Same seeded vector, ComplexF64 and ComplexF32."""
+    import edkit_hip as edk
+    from edkit_hip import _lib
+    from edkit_hip.matvec import device_operator
+    B = getattr(edk, kind)(L=32, **kw)
+    H = xxz(B)
+    op = device_operator(H)
+    for dt, tol in ((np.complex128, 1e-13), (np.complex64, 1e-6)):
+        x = edk.DeviceArray.from_host(rand_state(B.D, 21, dt))
+        op.set_variant(dt, 1, _lib.EDK_VARIANT_TILE)
+        yt = (H @ x).to_host()
+        op.set_variant(dt, 1, _lib.EDK_VARIANT_GENERIC)
+        yg = (H @ x).to_host()
+        op.set_variant(dt, 1, _lib.EDK_VARIANT_AUTO)
+        assert np.linalg.norm(yt - yg) <= tol * np.linalg.norm(yg), (kind, dt)
