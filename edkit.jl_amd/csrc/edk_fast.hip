@@ -143,18 +143,18 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
   __shared__ double sV[FAST_MAXFAM * FAST_MAXENT];
   __shared__ uint32_t tgt[BAL_CAP];
   __shared__ uint8_t ent[BAL_CAP];
-  __shared__ __align__(16) ST contr[2][BAL_CAP];
+  __shared__ VT contr[BAL_CAP];  // contributions, (re, im) together
   __shared__ uint32_t wsum[BAL_TPB / 64];
   __shared__ uint32_t ndef;
   // L = 32, phase mode 0: run-length candidates, with the few targets that
   // have more than two longest zero runs deferred to a block-wide full search.
   // Results go back to LDS: the representative over the target and the
-  // coefficient into the target's own slot of contr[1] (only its owner reads
-  // it, then overwrites it with the contribution); the deferred queue lives
-  // in contr[0], dead before the gathers write there
+  // coefficient into the imaginary half of the target's own contr slot (only
+  // its owner reads it, then overwrites it with the contribution); the
+  // deferred queue lives in the real halves, dead before the gathers write
   constexpr bool RUNS = (LT == 32 && PH == 0 && KIND != EDK_PROJECTED);
-  uint16_t *dq = reinterpret_cast<uint16_t *>(&contr[0][0]);
-  static_assert(2 * BAL_CAP <= sizeof(contr[0]), "deferred queue must fit in contr[0]");
+  constexpr int DQS = sizeof(VT) / sizeof(uint32_t);  // queue entry d at the real half of contr[d]
+  uint32_t *dq = reinterpret_cast<uint32_t *>(contr);
   const FastTab &T = *A.tab;
   const int tid = threadIdx.x;
   if (RUNS && tid == 0) ndef = 0;
@@ -230,20 +230,20 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
         if (canon32_runs(tgt[j], a, b, per)) {
           const int code = (KIND == EDK_TRANSLATIONAL) ? per - 1 : (a == b ? 32 : 0) + per - 1;
           tgt[j] = (KIND == EDK_TRANSLATIONAL) ? a : min(a, b);
-          contr[1][j] = (ST)(sV[ent[j]] * sR[code]);
+          contr[j].y = (ST)(sV[ent[j]] * sR[code]);
         } else {
-          dq[atomicAdd(&ndef, 1u)] = (uint16_t)j;
+          dq[DQS * atomicAdd(&ndef, 1u)] = j;
         }
       }
     }
     __syncthreads();
     const uint32_t nd = ndef;
     for (uint32_t d = tid; d < nd; d += BAL_TPB) {
-      const uint32_t j = dq[d];
+      const uint32_t j = dq[DQS * d];
       uint32_t r, c;
       canon32_brute<KIND>(tgt[j], r, c);
       tgt[j] = r;
-      contr[1][j] = (ST)(sV[ent[j]] * sR[c]);
+      contr[j].y = (ST)(sV[ent[j]] * sR[c]);
     }
     // results visible; the queue is dead before the contributions reuse it
     __syncthreads();
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
       const uint32_t j = tid + BAL_TPB * q;
       ok[q] = j < total;
       rep[q] = ok[q] ? tgt[j] : 0u;
-      cfr[q] = ok[q] ? contr[1][j] : (ST)0;
+      cfr[q] = ok[q] ? contr[j].y : (ST)0;
       lo[q] = hi[q] = 0;
       if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
     }
@@ -292,8 +292,10 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
       }
     }
     if (j < total) {
-      contr[0][j] = cr_;
-      contr[1][j] = ci_;
+      VT cv;
+      cv.x = cr_;
+      cv.y = ci_;
+      contr[j] = cv;
     }
   }
   __syncthreads();
@@ -310,8 +312,9 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
     }
     ST orr = 0, oi = 0;
     for (uint32_t q = off; q < off + n; q++) {
-      orr += contr[0][q];
-      oi += contr[1][q];
+      const VT cv = contr[q];
+      orr += cv.x;
+      oi += cv.y;
     }
     ST yr = dg * (ST)xi.x + orr * invR;
     ST yi = dg * (ST)xi.y + oi * invR;
