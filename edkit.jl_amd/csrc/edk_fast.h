@@ -431,6 +431,61 @@ __device__ __forceinline__ uint32_t region_find(const FastArgs &A, uint32_t key,
   return HASH_EMPTY;
 }
 
+// Branch-free lookups of BAL_K targets at once, so each step's loads for all
+// targets issue back to back instead of one target's load waiting on the
+// previous one's branch.  Invalid targets read slot 0 / key 0 (always in
+// range; the key table is padded) and are masked afterwards.
+#define EDK_NEED_BS 0xfffffffeu
+template <int K>
+__device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint32_t (&rep)[K], const bool (&ok)[K],
+                                                   uint32_t (&jj)[K]) {
+  uint2 rg[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) rg[q] = A.region[(ok[q] ? rep[q] : 0u) >> A.rshift];
+  uint32_t lo[K], hi[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const uint32_t key = ok[q] ? rep[q] : 0u;
+    const uint32_t sh = rg[q].y & 31, foff = rg[q].y >> 5;
+    const uint32_t fb = (key & ((1u << A.rshift) - 1)) >> sh;
+    lo[q] = rg[q].x + A.fine[foff + fb];
+    hi[q] = rg[q].x + A.fine[foff + fb + 1];
+  }
+  edk_u32x4 kv[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) __builtin_memcpy(&kv[q], A.keys + lo[q], sizeof(edk_u32x4));
+  bool any_bs = false;
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const uint32_t key = rep[q], n = hi[q] - lo[q];
+    const uint32_t below = (kv[q].x < key) + (n > 1 && kv[q].y < key) + (n > 2 && kv[q].z < key) +
+                           (n > 3 && kv[q].w < key);
+    const bool eq = (n > 0 && kv[q].x == key) || (n > 1 && kv[q].y == key) || (n > 2 && kv[q].z == key) ||
+                    (n > 3 && kv[q].w == key);
+    jj[q] = !ok[q] ? HASH_EMPTY : (n > FINE_SCAN ? EDK_NEED_BS : (eq ? lo[q] + below : HASH_EMPTY));
+    any_bs = any_bs || jj[q] == EDK_NEED_BS;
+  }
+  if (any_bs) {
+    // buckets of more than FINE_SCAN keys (rare): binary search
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+      if (jj[q] == EDK_NEED_BS) {
+        uint32_t a = lo[q], b = hi[q], r = HASH_EMPTY;
+        while (a < b) {
+          const uint32_t mid = (a + b) >> 1, v = A.keys[mid];
+          if (v == rep[q]) {
+            r = mid;
+            break;
+          }
+          if (v < rep[q]) a = mid + 1;
+          else b = mid;
+        }
+        jj[q] = r;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // balanced kernel: one block per chunk of whole rows holding at most
 // BAL_CAP targets.  Phase 1: the row threads write their targets (and entry

@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
   }
   __syncthreads();
   // phase 2: BAL_K targets per thread, no divergence
-  uint32_t rep[BAL_K], lo[BAL_K], hi[BAL_K], jj[BAL_K];
+  uint32_t rep[BAL_K], jj[BAL_K];
   bool ok[BAL_K];
   ST cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
   if constexpr (RUNS) {
@@ -253,8 +253,6 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
       ok[q] = j < total;
       rep[q] = ok[q] ? tgt[j] : 0u;
       cfr[q] = ok[q] ? contr[j].y : (ST)0;
-      lo[q] = hi[q] = 0;
-      if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
     }
   } else {
 #pragma unroll
@@ -268,28 +266,32 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
         if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
         else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
       }
-      lo[q] = hi[q] = 0;
-      if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
       double v = ok[q] ? sV[ent[j]] : 0.0;
       cfr[q] = (ST)(v * cr);
       if (CPLX) cfi[q] = (ST)(v * ci);
     }
   }
+  region_lookup_many<BAL_K>(A, rep, ok, jj);
+  // gathers issued together; an absent target reads x[0] and drops it by
+  // selection, so a non-finite x[0] cannot leak into the sum
+  VT xg[BAL_K];
 #pragma unroll
-  for (int q = 0; q < BAL_K; q++) jj[q] = ok[q] ? region_find(A, rep[q], lo[q], hi[q]) : HASH_EMPTY;
+  for (int q = 0; q < BAL_K; q++) xg[q] = x[jj[q] != HASH_EMPTY ? jj[q] : 0u];
 #pragma unroll
   for (int q = 0; q < BAL_K; q++) {
     const uint32_t j = tid + BAL_TPB * q;
     ST cr_ = 0, ci_ = 0;
-    if (jj[q] != HASH_EMPTY) {
-      VT xj = x[jj[q]];
-      if (CPLX) {
-        cr_ = cfr[q] * (ST)xj.x - cfi[q] * (ST)xj.y;
-        ci_ = cfr[q] * (ST)xj.y + cfi[q] * (ST)xj.x;
-      } else {
-        cr_ = cfr[q] * (ST)xj.x;
-        ci_ = cfr[q] * (ST)xj.y;
-      }
+    const VT xj = xg[q];
+    if (CPLX) {
+      cr_ = cfr[q] * (ST)xj.x - cfi[q] * (ST)xj.y;
+      ci_ = cfr[q] * (ST)xj.y + cfi[q] * (ST)xj.x;
+    } else {
+      cr_ = cfr[q] * (ST)xj.x;
+      ci_ = cfr[q] * (ST)xj.y;
+    }
+    if (jj[q] == HASH_EMPTY) {
+      cr_ = 0;
+      ci_ = 0;
     }
     if (j < total) {
       VT cv;
