@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
     s = A.keys[r0 + tid];
     const uint32_t ns = ~s & A.mask;
     for (int f = 0; f < T.nfam; f++) {
-      uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      uint32_t r = LT == 32 ? __builtin_amdgcn_alignbit(s, s, 32 - T.fam[f].d) : rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
       for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
     }
   }
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
     uint32_t o = off;
     for (int f = 0; f < T.nfam; f++) {
       const int d = T.fam[f].d;
-      uint32_t r = rotr32(s, A.L - d, A.L, A.mask);
+      uint32_t r = LT == 32 ? __builtin_amdgcn_alignbit(s, s, 32 - d) : rotr32(s, A.L - d, A.L, A.mask);
       for (int e = 0; e < T.fam[f].nent; e++) {
         const int flip = T.fam[f].ent[e].flip;
         uint32_t bits = ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask);
@@ -307,13 +307,27 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
     const uint32_t ns = ~s & A.mask;
     ST dg = 0;
     for (int f = 0; f < T.nfam; f++) {
-      uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      uint32_t r = LT == 32 ? __builtin_amdgcn_alignbit(s, s, 32 - T.fam[f].d) : rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
       uint32_t nr = ~r & A.mask;
       dg += (ST)(T.fam[f].diag[0] * __popc(ns & nr) + T.fam[f].diag[1] * __popc(ns & r) +
                  T.fam[f].diag[2] * __popc(s & nr) + T.fam[f].diag[3] * __popc(s & r));
     }
     ST orr = 0, oi = 0;
-    for (uint32_t q = off; q < off + n; q++) {
+    // in generator order, four reads in flight at a time
+    uint32_t q = off;
+    const uint32_t qe = off + n;
+    for (; q + 4 <= qe; q += 4) {
+      const VT c0 = contr[q], c1 = contr[q + 1], c2 = contr[q + 2], c3 = contr[q + 3];
+      orr += c0.x;
+      oi += c0.y;
+      orr += c1.x;
+      oi += c1.y;
+      orr += c2.x;
+      oi += c2.y;
+      orr += c3.x;
+      oi += c3.y;
+    }
+    for (; q < qe; q++) {
       const VT cv = contr[q];
       orr += cv.x;
       oi += cv.y;

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
   }
   __syncthreads();
   // phase 2: BAL_K targets per thread -> (position, coefficient)
-  uint32_t rep[BAL_K], lo[BAL_K], hi[BAL_K];
+  uint32_t rep[BAL_K], jj[BAL_K];
   bool ok[BAL_K];
   ST cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
 #pragma unroll
@@ -97,18 +97,16 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
       if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
       else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
     }
-    lo[q] = hi[q] = 0;
-    if (ok[q]) region_bounds(A, rep[q], lo[q], hi[q]);
     double v = ok[q] ? sV[ent[j]] : 0.0;
     cfr[q] = (ST)(v * cr);
     if (CPLX) cfi[q] = (ST)(v * ci);
   }
+  region_lookup_many<BAL_K>(A, rep, ok, jj);
 #pragma unroll
   for (int q = 0; q < BAL_K; q++) {
     const uint32_t j = tid + BAL_TPB * q;
-    const uint32_t jj = ok[q] ? region_find(A, rep[q], lo[q], hi[q]) : HASH_EMPTY;
     if (j < total) {
-      tgt[j] = jj;
+      tgt[j] = jj[q];
       tcr[j] = cfr[q];
       if (CPLX) tci[j] = cfi[q];
     }
