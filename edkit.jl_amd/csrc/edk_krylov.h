@@ -4,6 +4,8 @@
 #include "edk_matvec.h"
 
 #define EDK_KRYLOV_MAXM 1024
+// slot of d_s / h_s that holds a norm while d_s[0..j) holds coefficients
+#define NORM_SLOT (EDK_KRYLOV_MAXM + 1)
 
 // Deterministic FP64 vector kernels over length-n vectors of one dtype.
 struct KrylovWork {
@@ -22,7 +24,9 @@ struct KrylovWork {
   int norm(const void *a, double *out);
   int axpy(void *y, const void *x, double2 alpha);
   int scale(void *y, const void *x, double s);
-  int reorth(const void *V, int64_t ldv, int j, void *w);
+  int cgs_sweep(const void *V, int64_t ldv, int j, void *w);
+  // reorthogonalise w against V[:, :j] (one or two sweeps) and return ||w||
+  int reorth_norm(const void *V, int64_t ldv, int j, void *w, double *wnorm, int *sweeps);
   int combine(const void *V, int64_t ldv, int j, const double2 *c, double scale, void *out, int out_dtype);
 };
 

@@ -198,7 +198,9 @@ int KrylovWork::init(edk_ctx *c, int64_t n_, int dtype_, int maxm, int slot) {
   if (maxm + 2 > EDK_PINNED_SLOT) return edk_fail(EDK_ERR_UNSUPPORTED, "Krylov dimension above 1024");
   size_t pb = (size_t)nblk * (maxm + 2) * sizeof(double2);
   if (edk_dalloc(ctx, (void **)&d_part, pb) != hipSuccess) return edk_fail(EDK_ERR_OOM, "krylov partials");
-  if (edk_dalloc(ctx, (void **)&d_s, (maxm + 2) * sizeof(double2)) != hipSuccess)
+  // coefficients in d_s[0..maxm+1], a norm in d_s[NORM_SLOT]
+  static_assert(NORM_SLOT < EDK_PINNED_SLOT, "norm slot must fit in the pinned buffer");
+  if (edk_dalloc(ctx, (void **)&d_s, (NORM_SLOT + 1) * sizeof(double2)) != hipSuccess)
     return edk_fail(EDK_ERR_OOM, "krylov coefficients");
   h_s = (double2 *)ctx->pinned + (size_t)slot * EDK_PINNED_SLOT;
   return EDK_OK;
@@ -259,16 +261,47 @@ int KrylovWork::scale(void *y, const void *x, double s) {
   return EDK_OK;
 }
 
-// w -= V[:, :j] (V[:, :j]^H w), twice
-int KrylovWork::reorth(const void *V, int64_t ldv, int j, void *w) {
+// One classical Gram-Schmidt sweep: h = V[:, :j]^H w, w -= V[:, :j] h.
+int KrylovWork::cgs_sweep(const void *V, int64_t ldv, int j, void *w) {
   hipStream_t s = ctx->stream;
-  for (int sweep = 0; sweep < 2; sweep++) {
-    DISPATCH(VT, k_gemv_t<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)V, ldv, j, (const VT *)w, n, d_part));
-    k_reduce_part<<<j, RED_TPB, 0, s>>>(d_part, nblk, d_s);
-    unsigned g = (unsigned)((n + RED_TPB - 1) / RED_TPB);
-    DISPATCH(VT, k_gemv_n<VT, VT><<<g, RED_TPB, 0, s>>>((const VT *)V, ldv, j, d_s, (VT *)w, n, 0, 0.0));
-  }
+  DISPATCH(VT, k_gemv_t<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)V, ldv, j, (const VT *)w, n, d_part));
+  k_reduce_part<<<j, RED_TPB, 0, s>>>(d_part, nblk, d_s);
+  unsigned g = (unsigned)((n + RED_TPB - 1) / RED_TPB);
+  DISPATCH(VT, k_gemv_n<VT, VT><<<g, RED_TPB, 0, s>>>((const VT *)V, ldv, j, d_s, (VT *)w, n, 0, 0.0));
   EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+
+// Reorthogonalise w against V[:, :j] and return ||w||: a first CGS sweep,
+// then a second one only when the first removed a large part of w (the
+// criterion of Daniel, Gragg, Kaufman and Stewart: ||w'|| < ||w|| / sqrt(2),
+// with ||w||^2 = ||w'||^2 + ||h||^2 since w' is orthogonal to V h).  In a
+// Lanczos step w is already nearly orthogonal to V, so the second sweep --
+// and its two passes over V -- is rarely needed; when it is, two sweeps
+// leave w orthogonal to working precision ("twice is enough").
+int KrylovWork::reorth_norm(const void *V, int64_t ldv, int j, void *w, double *wnorm, int *sweeps) {
+  hipStream_t s = ctx->stream;
+  if (int rc = cgs_sweep(V, ldv, j, w)) return rc;
+  // h back to the host while the norm of w' is reduced (one synchronisation)
+  EDK_HIP(hipMemcpyAsync(h_s, d_s, (size_t)j * sizeof(double2), hipMemcpyDeviceToHost, s));
+  DISPATCH(VT, k_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)w, (const VT *)nullptr, n, d_part));
+  k_reduce_part<<<1, RED_TPB, 0, s>>>(d_part, nblk, d_s + NORM_SLOT);
+  EDK_HIP(hipGetLastError());
+  EDK_HIP(hipMemcpyAsync(h_s + NORM_SLOT, d_s + NORM_SLOT, sizeof(double2), hipMemcpyDeviceToHost, s));
+  EDK_HIP(hipStreamSynchronize(s));
+  double hh = 0.0;
+  for (int c = 0; c < j; c++) hh += h_s[c].x * h_s[c].x + h_s[c].y * h_s[c].y;
+  double w2 = h_s[NORM_SLOT].x;
+  int nsw = 1;
+  if (w2 < 0.5 * (w2 + hh)) {
+    if (int rc = cgs_sweep(V, ldv, j, w)) return rc;
+    double nrm;
+    if (int rc = norm(w, &nrm)) return rc;
+    w2 = nrm * nrm;
+    nsw = 2;
+  }
+  *wnorm = std::sqrt(w2);
+  if (sweeps) *sweeps = nsw;
   return EDK_OK;
 }
 
@@ -382,9 +415,8 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
     double aj = d.x;
     alpha[j - 1] = aj;
     if (int rc = K.axpy(w, vj, make_double2(-aj, 0.0))) return -rc;
-    if (int rc = K.reorth(V, ldv, j, w)) return -rc;
     double bj;
-    if (int rc = K.norm(w, &bj)) return -rc;
+    if (int rc = K.reorth_norm(V, ldv, j, w, &bj, nullptr)) return -rc;
     beta[j - 1] = bj;
     const double sq = 1.4901161193847656e-08;  // sqrt(eps(Float64))
     if (bj <= sq * std::max(1.0, std::fabs(aj))) {
