@@ -305,6 +305,100 @@ __device__ __forceinline__ bool canon32_runs(uint32_t s, uint32_t &minA, uint32_
   return true;
 }
 
+// Run-length candidates for any L with 16 <= L <= 32 and unit cell 1, every
+// phase mode: the same longest-zero-run argument on L-bit words (LT == 32:
+// one v_alignbit per rotation; otherwise a funnel rotation in L bits).  The
+// rotation count of the chosen candidate gives the phase (first minimum, as
+// the reference's loops: ties between the two candidates of a period-L/2
+// word take the smaller count).  Returns 1 with rep and the factor
+// conj(phase) * R (cr, ci) exactly as fast_canon / fast_canon32 compute it,
+// 0 when the representative cannot be in the sector, -1 when the word has
+// more than two longest runs (the caller runs the full search).
+template <int LT>
+__device__ __forceinline__ uint32_t rotr_w(uint32_t x, uint32_t m, int L, uint32_t mask) {
+  if (LT == 32) return __builtin_amdgcn_alignbit(x, x, m);
+  return rotr32(x, (int)m, L, mask);  // 0 <= m < L
+}
+template <int LT>
+__device__ __forceinline__ uint32_t rotl_w(uint32_t x, uint32_t m, int L, uint32_t mask) {
+  if (LT == 32) return __builtin_amdgcn_alignbit(x, x, (32u - m) & 31u);
+  return rotr32(x, m == 0 ? 0 : L - (int)m, L, mask);  // 0 <= m < L
+}
+template <int KIND, int PH, int LT>
+__device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci) {
+  const int L = (LT == 32) ? 32 : B.L;
+  const uint32_t mask = (LT == 32) ? 0xffffffffu : B.mask;
+  const uint32_t z1 = ~s & mask;
+  const uint32_t z2 = z1 & rotl_w<LT>(z1, 1, L, mask);
+  const uint32_t z4 = z2 & rotl_w<LT>(z2, 2, L, mask);
+  const uint32_t z8 = z4 & rotl_w<LT>(z4, 4, L, mask);
+  const uint32_t z16 = z8 & rotl_w<LT>(z8, 8, L, mask);
+  uint32_t acc = z16 ? z16 : mask;
+  uint32_t len = z16 ? 16u : 0u;
+#define EDK_RUNSTEP(ZM, M)                                                             \
+  {                                                                                    \
+    const uint32_t sh_ = (LT == 32 || len < (uint32_t)L) ? len : len - (uint32_t)L;    \
+    const uint32_t c_ = acc & rotl_w<LT>(ZM, sh_, L, mask);                            \
+    const bool t_ = c_ != 0;                                                           \
+    acc = t_ ? c_ : acc;                                                               \
+    len = t_ ? len + (M) : len;                                                        \
+  }
+  EDK_RUNSTEP(z8, 8u)
+  EDK_RUNSTEP(z4, 4u)
+  EDK_RUNSTEP(z2, 2u)
+  EDK_RUNSTEP(z1, 1u)
+#undef EDK_RUNSTEP
+  if (__popc(acc) > 2) return -1;
+  const uint32_t p1 = __builtin_ctz(acc);  // acc != 0: it only takes non-zero values
+  const uint32_t m2 = acc & (acc - 1);
+  const uint32_t p2 = m2 ? __builtin_ctz(m2) : p1;
+  // forward images: rotate right by ia = (p + 1) mod L
+  const uint32_t ia1 = (p1 + 1 == (uint32_t)L) ? 0u : p1 + 1, ia2 = (p2 + 1 == (uint32_t)L) ? 0u : p2 + 1;
+  const uint32_t va1 = rotr_w<LT>(s, ia1, L, mask), va2 = rotr_w<LT>(s, ia2, L, mask);
+  const uint32_t minA = min(va1, va2);
+  int period = L;
+  if ((L & 1) == 0 && rotl_w<LT>(s, (uint32_t)(L >> 1), L, mask) == s) period = L >> 1;
+  if (KIND == EDK_TRANSLATIONAL) {
+    rep = minA;
+    // check_momentum: period * k' divisible by L, period in {L, L/2}
+    if (period != L && (B.kp & 1)) return 0;
+    const double R = B.Rtab[period - 1];
+    if (PH != 0) {
+      const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
+      const double2 c = B.ph[Ma];
+      cr = c.x * R;
+      ci = -c.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    return 1;
+  }
+  // TranslationParityBasis: reflected images rev(rot_right^k(s)) = rotl^k(rev(s)),
+  // k = (p + 1 - len) mod L
+  const uint32_t rb = (LT == 32) ? __brev(s) : rev32(s, L);
+  int ib1 = (int)p1 + 1 - (int)len, ib2 = (int)p2 + 1 - (int)len;
+  ib1 += ib1 < 0 ? L : 0;
+  ib1 += ib1 < 0 ? L : 0;
+  ib2 += ib2 < 0 ? L : 0;
+  ib2 += ib2 < 0 ? L : 0;
+  const uint32_t vb1 = rotl_w<LT>(rb, (uint32_t)ib1, L, mask), vb2 = rotl_w<LT>(rb, (uint32_t)ib2, L, mask);
+  const uint32_t minB = min(vb1, vb2);
+  const bool qp = (minA == minB);
+  const double R = B.Rtab[(qp ? L : 0) + period - 1];
+  double sgn = 1.0;
+  if (PH != 0) {
+    const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
+    const int Mb = (vb1 == minB && (vb2 != minB || ib1 < ib2)) ? ib1 : ib2;
+    if (minB < minA) sgn = (double)B.P * ((B.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (B.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
+  }
+  rep = minA < minB ? minA : minB;
+  cr = sgn * R;
+  ci = 0.0;
+  return 1;
+}
+
 // Full search for phase mode 0 at L = 32: representative and the index of its
 // norm in the staged norm table (period - 1, plus 32 when the reflection
 // maps the orbit onto itself for TranslationParityBasis).

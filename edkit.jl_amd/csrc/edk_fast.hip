@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
 }
 
 template <int KIND, int PH, int LT, typename VT, typename ST>
-__global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_PROJECTED) ? 7 : 1) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
+__global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && KIND != EDK_PROJECTED) ? 7 : 1) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate) {
   constexpr bool CPLX = (PH == 2);
   __shared__ double sR[2 * 64];
@@ -152,9 +152,12 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
   // coefficient into the imaginary half of the target's own contr slot (only
   // its owner reads it, then overwrites it with the contribution); the
   // deferred queue lives in the real halves, dead before the gathers write
-  constexpr bool RUNS = (LT == 32 && PH == 0 && KIND != EDK_PROJECTED);
+  // (LT == 32: L = 32; LT == 1: 16 <= L < 32; both with unit cell 1)
+  constexpr bool RUNS = (LT == 32 || LT == 1) && KIND != EDK_PROJECTED;
   constexpr int DQS = sizeof(VT) / sizeof(uint32_t);  // queue entry d at the real half of contr[d]
   uint32_t *dq = reinterpret_cast<uint32_t *>(contr);
+  // complex coefficients fill their contr slot: the queue then has its own array
+  __shared__ uint16_t dqc[(RUNS && CPLX) ? BAL_CAP : 1];
   const FastTab &T = *A.tab;
   const int tid = threadIdx.x;
   if (RUNS && tid == 0) ndef = 0;
@@ -220,39 +223,72 @@ __global__ void __launch_bounds__(BAL_TPB, (LT == 32 && PH == 0 && KIND != EDK_P
   bool ok[BAL_K];
   ST cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
   if constexpr (RUNS) {
-    // one target at a time, results straight to LDS: little live state
+    // one target at a time, results straight to LDS: little live state.  A
+    // representative outside the sector is marked by ent = 0xff.
 #pragma unroll 1
     for (int q = 0; q < BAL_K; q++) {
       const uint32_t j = tid + BAL_TPB * q;
       if (j < total) {
-        uint32_t a, b;
-        int per;
-        if (canon32_runs(tgt[j], a, b, per)) {
-          const int code = (KIND == EDK_TRANSLATIONAL) ? per - 1 : (a == b ? 32 : 0) + per - 1;
-          tgt[j] = (KIND == EDK_TRANSLATIONAL) ? a : min(a, b);
-          contr[j].y = (ST)(sV[ent[j]] * sR[code]);
+        uint32_t r;
+        double cr = 0.0, ci = 0.0;
+        const int st = canon_runs<KIND, PH, LT>(B, tgt[j], r, cr, ci);
+        if (st > 0) {
+          tgt[j] = r;
+          const double v = sV[ent[j]];
+          if (CPLX) {
+            VT c;
+            c.x = (ST)(v * cr);
+            c.y = (ST)(v * ci);
+            contr[j] = c;
+          } else {
+            contr[j].y = (ST)(v * cr);
+          }
+        } else if (st == 0) {
+          ent[j] = 0xff;
         } else {
-          dq[DQS * atomicAdd(&ndef, 1u)] = j;
+          const uint32_t d = atomicAdd(&ndef, 1u);
+          if (CPLX) dqc[d] = (uint16_t)j;
+          else dq[DQS * d] = j;
         }
       }
     }
     __syncthreads();
     const uint32_t nd = ndef;
     for (uint32_t d = tid; d < nd; d += BAL_TPB) {
-      const uint32_t j = dq[DQS * d];
-      uint32_t r, c;
-      canon32_brute<KIND>(tgt[j], r, c);
-      tgt[j] = r;
-      contr[j].y = (ST)(sV[ent[j]] * sR[c]);
+      const uint32_t j = CPLX ? (uint32_t)dqc[d] : dq[DQS * d];
+      uint32_t r = 0;
+      double cr = 0.0, ci = 0.0;
+      const bool okb = (LT == 32) ? fast_canon32<KIND, PH>(B, tgt[j], r, cr, ci)
+                                  : fast_canon<KIND, PH>(B, tgt[j], r, cr, ci);
+      if (okb) {
+        tgt[j] = r;
+        const double v = sV[ent[j]];
+        if (CPLX) {
+          VT c;
+          c.x = (ST)(v * cr);
+          c.y = (ST)(v * ci);
+          contr[j] = c;
+        } else {
+          contr[j].y = (ST)(v * cr);
+        }
+      } else {
+        ent[j] = 0xff;
+      }
     }
     // results visible; the queue is dead before the contributions reuse it
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < BAL_K; q++) {
       const uint32_t j = tid + BAL_TPB * q;
-      ok[q] = j < total;
+      ok[q] = j < total && ent[j] != 0xff;
       rep[q] = ok[q] ? tgt[j] : 0u;
-      cfr[q] = ok[q] ? contr[j].y : (ST)0;
+      if (CPLX) {
+        const VT c = contr[j];
+        cfr[q] = ok[q] ? c.x : (ST)0;
+        cfi[q] = ok[q] ? c.y : (ST)0;
+      } else {
+        cfr[q] = ok[q] ? contr[j].y : (ST)0;
+      }
     }
   } else {
 #pragma unroll
@@ -730,7 +766,10 @@ static void launch_bal_lt(const FastArgs &A, int dtype, void *y, const void *x, 
 template <int KIND, int PH>
 static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
   if (A.chunk) {
+    // rotation flavour: L = 32 (v_alignbit), 16 <= L < 32 with unit cell 1
+    // (run-length canonicaliser on L-bit words), anything else (full search)
     if (A.L == 32 && A.A == 1) launch_bal_lt<KIND, PH, 32>(A, dtype, y, x, acc, st);
+    else if (A.A == 1 && A.L >= 16 && KIND != EDK_PROJECTED) launch_bal_lt<KIND, PH, 1>(A, dtype, y, x, acc, st);
     else launch_bal_lt<KIND, PH, 0>(A, dtype, y, x, acc, st);
     return;
   }

@@ -20,8 +20,14 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
   __shared__ uint32_t roff[BAL_TPB], rcnt[BAL_TPB];
   __shared__ ST rdg[BAL_TPB], rinv[BAL_TPB];
   __shared__ uint32_t wsum[BAL_TPB / 64];
+  // run-length canonicaliser (unit cell 1, 16 <= L <= 32) with a block-wide
+  // full search for the targets it defers
+  constexpr bool RUNS = (LT == 32 || LT == 1) && KIND != EDK_PROJECTED;
+  __shared__ uint16_t dq[RUNS ? BAL_CAP : 1];
+  __shared__ uint32_t ndef;
   const FastTab &T = *A.tab;
   const int tid = threadIdx.x;
+  if (RUNS && tid == 0) ndef = 0;
   if (KIND != EDK_PROJECTED) {
     int nr = (KIND == EDK_TRANSLATION_PARITY ? 2 : 1) * A.len;
     if (tid < nr) sR[tid] = A.Rtab[tid];
@@ -86,20 +92,70 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
   uint32_t rep[BAL_K], jj[BAL_K];
   bool ok[BAL_K];
   ST cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
-#pragma unroll
-  for (int q = 0; q < BAL_K; q++) {
-    const uint32_t j = tid + BAL_TPB * q;
-    ok[q] = j < total;
-    uint32_t t = ok[q] ? tgt[j] : 0u;
-    double cr = 0.0, ci = 0.0;
-    rep[q] = 0;
-    if (ok[q]) {
-      if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
-      else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
+  if constexpr (RUNS) {
+    // one target at a time, results to LDS; ent = 0xff marks a representative
+    // outside the sector
+#pragma unroll 1
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      if (j < total) {
+        uint32_t r;
+        double cr = 0.0, ci = 0.0;
+        const int st = canon_runs<KIND, PH, LT>(B, tgt[j], r, cr, ci);
+        if (st > 0) {
+          tgt[j] = r;
+          const double v = sV[ent[j]];
+          tcr[j] = (ST)(v * cr);
+          if (CPLX) tci[j] = (ST)(v * ci);
+        } else if (st == 0) {
+          ent[j] = 0xff;
+        } else {
+          dq[atomicAdd(&ndef, 1u)] = (uint16_t)j;
+        }
+      }
     }
-    double v = ok[q] ? sV[ent[j]] : 0.0;
-    cfr[q] = (ST)(v * cr);
-    if (CPLX) cfi[q] = (ST)(v * ci);
+    __syncthreads();
+    const uint32_t nd = ndef;
+    for (uint32_t d = tid; d < nd; d += BAL_TPB) {
+      const uint32_t j = dq[d];
+      uint32_t r = 0;
+      double cr = 0.0, ci = 0.0;
+      const bool okb = (LT == 32) ? fast_canon32<KIND, PH>(B, tgt[j], r, cr, ci)
+                                  : fast_canon<KIND, PH>(B, tgt[j], r, cr, ci);
+      if (okb) {
+        tgt[j] = r;
+        const double v = sV[ent[j]];
+        tcr[j] = (ST)(v * cr);
+        if (CPLX) tci[j] = (ST)(v * ci);
+      } else {
+        ent[j] = 0xff;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      ok[q] = j < total && ent[j] != 0xff;
+      rep[q] = ok[q] ? tgt[j] : 0u;
+      cfr[q] = ok[q] ? tcr[j] : (ST)0;
+      if (CPLX) cfi[q] = ok[q] ? tci[j] : (ST)0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      ok[q] = j < total;
+      uint32_t t = ok[q] ? tgt[j] : 0u;
+      double cr = 0.0, ci = 0.0;
+      rep[q] = 0;
+      if (ok[q]) {
+        if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
+        else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
+      }
+      double v = ok[q] ? sV[ent[j]] : 0.0;
+      cfr[q] = (ST)(v * cr);
+      if (CPLX) cfi[q] = (ST)(v * ci);
+    }
   }
   region_lookup_many<BAL_K>(A, rep, ok, jj);
 #pragma unroll
@@ -159,6 +215,7 @@ static void launch_blk(const FastArgs &A, int dtype, void *y, const void *x, int
 template <int KIND, int PH, int NV>
 static void launch_blk_lt(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
   if (A.L == 32 && A.A == 1) launch_blk<KIND, PH, 32, NV>(A, dtype, y, x, acc, st);
+  else if (A.A == 1 && A.L >= 16 && KIND != EDK_PROJECTED) launch_blk<KIND, PH, 1, NV>(A, dtype, y, x, acc, st);
   else launch_blk<KIND, PH, 0, NV>(A, dtype, y, x, acc, st);
 }
 

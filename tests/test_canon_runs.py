@@ -57,3 +57,53 @@ def test_candidates_give_the_dihedral_minimum():
         if len(cands) <= 2:
             per = next(q for q in (1, 2, 4, 8, 16, 32) if rotl(s, q % 32) == s)
             assert per == (16 if rotr(s, 16) == s else 32), hex(s)
+
+
+def test_candidates_any_length_with_rotation_counts():
+    """The same argument on L-bit words, 16 <= L <= 32, with the rotation
+    counts that fix the phase: the first (smallest) right-rotation count
+    reaching the forward minimum, and the first count k with
+    reverse(rotr^k(s)) at the reflected minimum, as the reference's loops
+    find them; a word with at most two longest runs has period L or L/2."""
+    rnd = random.Random(11)
+    for L in range(16, 33):
+        mask = (1 << L) - 1
+
+        def rl(x, m):
+            m %= L
+            return ((x << m) | (x >> (L - m))) & mask if m else x
+
+        def rr(x, m):
+            return rl(x, (L - m) % L)
+
+        def rv(x):
+            return int(f"{x:0{L}b}"[::-1], 2)
+
+        words = [rnd.getrandbits(L) for _ in range(300)]
+        words += [sum(1 << p for p in rnd.sample(range(L), L // 2)) for _ in range(300)]
+        words += [sum((rnd.getrandbits(L // 2)) << (k * (L // 2)) for k in range(2))
+                  for _ in range(50)] if L % 2 == 0 else []
+        for s in words:
+            z = {1: ~s & mask}
+            for m in (2, 4, 8, 16):
+                z[m] = z[m // 2] & rl(z[m // 2], m // 2)
+            acc, ln = mask, 0
+            for m in (16, 8, 4, 2, 1):
+                c = acc & rl(z[m], ln % L)
+                if c:
+                    acc, ln = c, ln + m
+            cands = [p for p in range(L) if acc >> p & 1]
+            if len(cands) > 2:
+                continue
+            fa = [(rr(s, (p + 1) % L), (p + 1) % L) for p in cands]
+            fb = [(rl(rv(s), (p + 1 - ln) % L), (p + 1 - ln) % L) for p in cands]
+            minA = min(v for v, _ in fa)
+            Ma = min(k for v, k in fa if v == minA)
+            minB = min(v for v, _ in fb)
+            Mb = min(k for v, k in fb if v == minB)
+            imgs = [rr(s, k) for k in range(L)]
+            refl = [rv(rr(s, k)) for k in range(L)]
+            assert (minA, Ma) == (min(imgs), imgs.index(min(imgs))), (L, bin(s))
+            assert (minB, Mb) == (min(refl), refl.index(min(refl))), (L, bin(s))
+            per = next(p for p in range(1, L + 1) if L % p == 0 and rl(s, p) == s)
+            assert per == (L // 2 if L % 2 == 0 and rl(s, L // 2) == s else L), (L, bin(s))
