@@ -377,6 +377,125 @@ def c5_workload(args, ev):
     print(json.dumps(out))
 
 
+def c2_workload(args, ev):
+    """Symmetry-reduced Krylov evolution: J1-J2 ring L=24 J1=1 J2=0.5,
+    TranslationalBasis N=12 k=0 (D = 112 720); a seeded Gaussian psi evolved
+    to t = 0.1, 0.2, ..., 10 (100 outputs) through KrylovEvolutionCache with
+    Krylov dimension 30, in ComplexF64 and in ComplexF32 vectors with FP64
+    Krylov scalars.  The CPU leg runs the oracle's KrylovEvolutionCache on the
+    same input: its time is the baseline and its states are the reference for
+    the reported errors."""
+    import edkit_hip as edk
+    sys.path.insert(0, ROOT)
+    from tests.models import j1j2
+    B = edk.TranslationalBasis(L=24, N=12, k=0)
+    H = j1j2(B, 0.5)
+    D = B.D
+    rng = np.random.default_rng(24)
+    psi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    psi /= np.linalg.norm(psi)
+    ts = [0.1 * (i + 1) for i in range(100)]
+    kw = dict(m_init=30, m_max=30)
+    dev, states = {}, {}
+    for name, dt in (("c64", np.complex128), ("c32", np.complex64)):
+        outs = [edk.DeviceArray((D,), dt) for _ in ts]
+        x0 = edk.DeviceArray.from_host(psi.astype(dt))
+        ev.device_sync()
+        t0 = time.perf_counter()
+        cache = edk.KrylovEvolutionCache(H, x0, dtype=dt, **kw)
+        for t, o in zip(ts, outs):
+            cache.evolve(t, out=o)
+        ev.device_sync()
+        secs = time.perf_counter() - t0
+        d = cache.diagnostics
+        states[name] = np.stack([o.to_host().astype(np.complex128) for o in outs])
+        dev[name] = {"s": secs, "outputs_per_s": len(ts) / secs, "matvecs": d["matvecs"],
+                     "restarts": d["restarts"], "max_dim": d["max_dim_used"]}
+        del cache, outs
+    out = {
+        "metric": "exp(-iHt)psi outputs/s (100 times, Krylov dim 30)",
+        "value": dev["c64"]["outputs_per_s"], "unit": "outputs/s", "n_gpus": 1, "steps": len(ts),
+        "warmup": 0, "ms_per_step": 1e3 * dev["c64"]["s"] / len(ts), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic (seeded Gaussian psi)",
+        "config": {"workload": "J1-J2 ring L=24 J2=0.5, TranslationalBasis N=12 k=0, exp(-iHt)psi to t=10 "
+                               "at dt=0.1, Krylov dim 30", "D": D, "parallelism": "single GPU"},
+        "device": dev,
+        "c32_vs_c64_max_err": float(np.max(np.linalg.norm(states["c32"] - states["c64"], axis=1))),
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        nthreads = min(16, os.cpu_count() or 1)
+        ob = orc.Basis("translational", 24, N=12, k=0, nthreads=nthreads)
+        op = orc.Op(ob, H.flatten())
+        t0 = time.perf_counter()
+        kc = orc.Krylov(op, psi, nthreads=nthreads, **kw)
+        ref = np.stack([kc.evolve(t) for t in ts])
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(ts) / secs, "unit": "outputs/s", "cores": nthreads, "kind": "port",
+                               "sample": f"the full workload: 100 outputs of the oracle's KrylovEvolutionCache "
+                                         f"(threaded matvec), {secs:.2f} s"}
+        out["c64_vs_oracle_max_err"] = float(np.max(np.linalg.norm(states["c64"] - ref, axis=1)))
+        out["c32_vs_oracle_max_err"] = float(np.max(np.linalg.norm(states["c32"] - ref, axis=1)))
+    print(json.dumps(out))
+
+
+def c1_workload(args, ev):
+    """Plain method, small: XXZ ring L=16 Jxy=1 Delta=1.5, TensorBasis
+    (D = 65 536), ComplexF64; a seeded Gaussian psi; 100 repeated H*psi, then a
+    200-step Lanczos for the lowest eigenvalue.  The CPU leg runs the same work
+    through the oracle (threaded scatter mul, its Lanczos)."""
+    import edkit_hip as edk
+    from edkit_hip.matvec import matvec_into
+    sys.path.insert(0, ROOT)
+    from tests.models import xxz
+    B = edk.TensorBasis(L=16)
+    H = xxz(B, 1.5)
+    D = B.D
+    rng = np.random.default_rng(16)
+    psi = rng.standard_normal(D) + 1j * rng.standard_normal(D)
+    psi /= np.linalg.norm(psi)
+    x = edk.DeviceArray.from_host(psi)
+    y = edk.DeviceArray((D,), np.complex128)
+    for _ in range(5):
+        matvec_into(y, H, x)
+    ev.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        matvec_into(y, H, x)
+    ev.device_sync()
+    t_mv = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E, r, steps, _ = edk.lanczos_ground(H, x, m=200, tol=0.0, refine=False)
+    t_lz = time.perf_counter() - t0
+    out = {
+        "metric": "100 H*psi + 200-step Lanczos, seconds", "value": t_mv + t_lz, "unit": "s", "n_gpus": 1,
+        "steps": 100, "warmup": 5, "ms_per_step": 1e3 * t_mv / 100, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic (seeded Gaussian psi)",
+        "config": {"workload": "XXZ ring L=16 Jxy=1 Delta=1.5, TensorBasis, 100 H*psi + 200-step Lanczos",
+                   "D": D, "parallelism": "single GPU"},
+        "matvec_s": t_mv, "lanczos_s": t_lz, "E0": E, "lanczos_resid": r,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        nthreads = min(16, os.cpu_count() or 1)
+        ob = orc.Basis("tensor", 16)
+        op = orc.Op(ob, H.flatten())
+        tgt = np.zeros(D, np.complex128)
+        t0 = time.perf_counter()
+        for _ in range(100):
+            tgt[:] = 0
+            op.mul_(tgt, psi, nthreads)
+        c_mv = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Eo, ro, so, _ = orc.lanczos_ground(op, psi, 200, 0.0, nthreads)
+        c_lz = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": c_mv + c_lz, "unit": "s", "cores": nthreads, "kind": "port",
+                               "sample": f"the full workload through the oracle: 100 threaded scatter H*psi "
+                                         f"{c_mv:.2f} s + 200-step Lanczos {c_lz:.2f} s"}
+        out["E0_vs_oracle"] = abs(E - Eo)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,9 +505,9 @@ def main():
     ap.add_argument("--c32", action="store_true", help="also time the ComplexF32 form")
     ap.add_argument("--copy-probe", action="store_true",
                     help="(kept for profiling commands; the copy probe always runs)")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "c5"],
-                    help="c3: L=32 H*psi roofline line (default); c4: batched uneven sectors; "
-                         "c5: serving stream")
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c3: L=32 H*psi roofline line (default); c1: L=16 plain method; c2: Krylov "
+                         "evolution L=24; c4: batched uneven sectors; c5: serving stream")
     ap.add_argument("--requests", type=int, default=2000, help="c5: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate, req/s")
     ap.add_argument("--workers", type=int, default=8, help="c5: server worker streams")
@@ -402,11 +521,8 @@ def main():
     import edkit_hip as edk
     from edkit_hip.matvec import device_operator
     ev = HipEvents()
-    if args.workload == "c4":
-        c4_workload(args, ev)
-        return
-    if args.workload == "c5":
-        c5_workload(args, ev)
+    if args.workload in ("c1", "c2", "c4", "c5"):
+        {"c1": c1_workload, "c2": c2_workload, "c4": c4_workload, "c5": c5_workload}[args.workload](args, ev)
         return
     t0 = time.perf_counter()
     B = edk.TranslationParityBasis(L=L, N=NUP, k=0, p=1)
