@@ -37,6 +37,7 @@ struct edk_op {
   double2 *d_val = nullptr;
   void *d_scratch = nullptr;
   void *fast = nullptr;       // state of the specialised kernel (edk_fast.hip)
+  void *csr = nullptr;        // assembled sector matrix (edk_assembly.hip), or null
   bool disable_fast = false;  // force the generic kernel (tests compare both)
   size_t scratch_bytes = 0;
   std::vector<int> h_tdesc, h_mrow_off, h_rowptr, h_col;
@@ -56,6 +57,9 @@ __device__ __forceinline__ uint64_t edk_content_to_packed_d(uint64_t c, int L, i
 
 // Specialised kernels; returns true when it handled the call.
 void edk_fast_release(edk_op *op);
+// assembled sector matrix: SpMM for nb > 1 (false: not assembled / not applicable)
+void edk_csr_release(edk_op *op);
+bool edk_matvec_csr(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);
 bool edk_matvec_fast(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);
 // schedule choice per (dtype, block width): forced, or timed by the autotuner
 int edk_fast_set_variant(edk_op *op, int dtype, int64_t nb, int variant);

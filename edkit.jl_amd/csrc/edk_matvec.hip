@@ -288,6 +288,7 @@ extern "C" int edk_operator_destroy(edk_op *op) {
   hipFree(op->d_val);
   hipFree(op->d_scratch);
   edk_fast_release(op);
+  edk_csr_release(op);
   if (op->basis) op->basis->refcount--;
   delete op;
   return EDK_OK;
@@ -314,6 +315,8 @@ int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
   MatvecArgs a = op->args();
   if (a.D == 0) return EDK_OK;
   unsigned grid = (unsigned)((a.D + 255) / 256);
+  // several right-hand sides with the assembled matrix cached (sparse!)
+  if (nb > 1 && edk_matvec_csr(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   for (int64_t c = 0; c < nb; c++) {
     if (dtype == EDK_C64)

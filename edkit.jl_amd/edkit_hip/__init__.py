@@ -13,7 +13,8 @@ from .basis import (AbelianBasis, AbstractBasis, ProjectedBasis, TensorBasis,  #
                     TranslationalBasis, TranslationParityBasis, basis, productstate)
 from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
-from .matvec import apply, device_operator, matvec_into, mul, mul_  # noqa: F401
+from .matvec import (Array, apply, clear_sparse_cache_, device_operator, matvec_into, mul,  # noqa: F401
+                     mul_, sparse, sparse_)
 from .operator import Operator, find_base, operator, trans_inv_operator  # noqa: F401
 from .serve import Server  # noqa: F401
 from .spin import spin  # noqa: F401

@@ -26,6 +26,7 @@ SYMBOLS = [
     "edk_basis_download", "edk_basis_index", "edk_basis_destroy",
     "edk_operator_create", "edk_operator_destroy", "edk_operator_count_offdiag",
     "edk_matvec", "edk_matvec_host",
+    "edk_operator_assemble", "edk_operator_download_csr", "edk_operator_release_csr",
     "edk_operator_set_variant", "edk_operator_autotune",
     "edk_batch_create", "edk_batch_launch", "edk_batch_destroy",
     "edk_server_create", "edk_server_submit", "edk_server_wait", "edk_server_clock",
@@ -133,6 +134,9 @@ def lib():
         "edk_operator_count_offdiag": (I32, [P, C.POINTER(C.c_int64)]),
         "edk_matvec": (I32, [P, I32, P, P, I64, I32]),
         "edk_matvec_host": (I32, [P, I32, P, P, I64, I32]),
+        "edk_operator_assemble": (I32, [P, C.POINTER(C.c_int64)]),
+        "edk_operator_download_csr": (I32, [P, P, P, P]),
+        "edk_operator_release_csr": (I32, [P]),
         "edk_operator_set_variant": (I32, [P, I32, I64, I32]),
         "edk_operator_autotune": (I32, [P, I32, I64, I32, C.POINTER(C.c_int), C.POINTER(D)]),
         "edk_batch_create": (I32, [P, I32, C.POINTER(BatchItem), I64, I32, PP]),

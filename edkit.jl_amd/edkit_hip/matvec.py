@@ -5,6 +5,7 @@ out); device vectors (DeviceArray or a tensor on the device) go straight to
 edk_matvec.  ComplexF64 vectors run the FP64 kernel, ComplexF32 vectors the
 FP32 kernel.  A 2-D (D, nb) array is a block of nb right-hand sides.
 """
+import collections
 import ctypes as C
 
 import numpy as np
@@ -38,6 +39,27 @@ class DeviceOperator:
         check(lib().edk_operator_autotune(self.h, dtype_code(np.dtype(dtype)), int(nb), int(reps),
                                           C.byref(v), C.byref(ms)))
         return v.value, ms.value
+
+    def assemble(self):
+        """Assemble (and cache) the sector matrix on the device; returns nnz."""
+        n = C.c_int64()
+        check(lib().edk_operator_assemble(self.h, C.byref(n)))
+        return n.value
+
+    def csr(self):
+        """The assembled sector matrix as scipy.sparse.csr_matrix (0-based)."""
+        import scipy.sparse as sp
+        nnz = self.assemble()
+        D = self.B.D
+        rp = np.zeros(D + 1, np.int64)
+        col = np.zeros(max(nnz, 1), np.int64)
+        val = np.zeros(2 * max(nnz, 1))
+        check(lib().edk_operator_download_csr(self.h, rp.ctypes.data, col.ctypes.data, val.ctypes.data))
+        v = val[: 2 * nnz].view(np.complex128)
+        return sp.csr_matrix((v, col[:nnz] - 1, rp - 1), shape=(D, D))
+
+    def release_csr(self):
+        check(lib().edk_operator_release_csr(self.h))
 
     def count_offdiag(self):
         n = C.c_int64()
@@ -110,6 +132,48 @@ def matvec_into(y, H, x, accumulate=False):
     nb = _nb(x.shape, H.B.D) if hasattr(x, "shape") else 1
     check(lib().edk_matvec(op.h, dtype_code(x.dtype), dev_ptr(y), dev_ptr(x), nb, int(accumulate)))
     return y
+
+
+def sparse(H):
+    """sparse(opt) (Operator.jl:210-216): the sector matrix, assembled on the device."""
+    return device_operator(H).csr()
+
+
+def Array(H):
+    """Array(opt) (Operator.jl:188-194): the dense sector matrix."""
+    return sparse(H).toarray()
+
+
+# sparse! keeps at most 8 assembled operators, least recently used evicted
+# (the reference's LRU{UInt, Any}(maxsize=8), Operator.jl:221)
+_SPARSE_CACHE = collections.OrderedDict()
+_SPARSE_CACHE_MAX = 8
+
+
+def sparse_(H):
+    """sparse!(opt) (Operator.jl:258-262): assemble and keep the matrix on the device;
+    later products with several right-hand sides use it.  Returns the matrix."""
+    op = device_operator(H)
+    S = op.csr()
+    _SPARSE_CACHE[id(op)] = op
+    _SPARSE_CACHE.move_to_end(id(op))
+    while len(_SPARSE_CACHE) > _SPARSE_CACHE_MAX:
+        _, old = _SPARSE_CACHE.popitem(last=False)
+        old.release_csr()
+    return S
+
+
+def clear_sparse_cache_(H=None):
+    """clear_sparse_cache!() (Operator.jl:272): drop the cached matrices (of H only,
+    when given)."""
+    if H is not None:
+        op = device_operator(H)
+        _SPARSE_CACHE.pop(id(op), None)
+        op.release_csr()
+        return
+    while _SPARSE_CACHE:
+        _, op = _SPARSE_CACHE.popitem()
+        op.release_csr()
 
 
 def mul(H, v):

@@ -150,6 +150,21 @@ int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
 int edk_matvec_host(edk_op *op, int dtype, void *y_host, const void *x_host,
                     int64_t nb, int accumulate);
 
+/* ---- explicit sector matrix --------------------------------------------------- */
+/* sparse(opt) / Array(opt) (Operator.jl:171-217) and the sparse!(opt) cache
+ * (:221-285): assemble the sector matrix on the device, row by row, with the
+ * gather kernel's matrix elements; each row sorted by column, duplicates
+ * summed, exact zeros dropped.  Once assembled (and until released, the
+ * reference's clear_sparse_cache!), edk_matvec with nb > 1 uses it (CSR SpMM);
+ * a single vector keeps the matrix-free kernels, as in the reference.
+ * edk_operator_assemble returns the number of stored entries. */
+int edk_operator_assemble(edk_op *op, int64_t *nnz);
+/* CSR of the sector matrix, 1-based: rowptr[D+1], col[nnz], val[2*nnz]
+ * (complex interleaved).  As SparseMatrixCSC(D, D, rowptr, col, val) this is
+ * the transpose; the binding takes copy(transpose(...)). */
+int edk_operator_download_csr(edk_op *op, int64_t *rowptr_1based, int64_t *col_1based, double *val);
+int edk_operator_release_csr(edk_op *op);
+
 /* ---- schedules and batching ------------------------------------------------ */
 /* No counterpart in the reference (its mul! has one CPU loop).  Schedules of
  * the specialised gather kernel, chosen per operator, dtype and block width:
