@@ -163,7 +163,9 @@ def c4_workload(args, ev):
     seeded complex vectors per sector, ComplexF32 matvec.  One step = H_s X_s
     for every sector s.  Timed four ways: one vector at a time (8 x 29
     launches), one block launch per sector (29), one graph launch, and one
-    graph launch after autotuning each sector's schedule."""
+    graph launch after autotuning each sector's schedule; then once more with
+    every sector's matrix assembled and cached (sparse! semantics: block
+    products through CSR SpMM).  value is the fastest graph mode."""
     import edkit_hip as edk
     from edkit_hip.matvec import device_operator, matvec_into
     sys.path.insert(0, ROOT)
@@ -227,8 +229,24 @@ def c4_workload(args, ev):
     batch = edk.MatvecBatch([(H, Xd, Yd) for H, Xd, Yd, _, _ in items])
     res["graph_autotuned_ms"] = timed(batch.launch, steps)
     batch.close()
+    # the assembled matrices (sparse! semantics): block products through CSR SpMM
+    ev.device_sync()
+    t0 = time.perf_counter()
+    for H, _, _, _, _ in items:
+        device_operator(H).assemble()
+    ev.device_sync()
+    res["assemble_all_s"] = time.perf_counter() - t0
+    res["assembled_block_per_sector_ms"] = timed(per_sector_block, steps)
+    batch = edk.MatvecBatch([(H, Xd, Yd) for H, Xd, Yd, _, _ in items])
+    res["graph_assembled_ms"] = timed(batch.launch, steps)
+    batch.close()
+    for H, _, _, _, _ in items:
+        device_operator(H).release_csr()
     from edkit_hip._lib import VARIANT_NAMES
-    best = min(res["graph_ms"], res["graph_autotuned_ms"])
+    modes = {"graph_ms": "one graph, matrix-free", "graph_autotuned_ms": "one graph, autotuned schedules",
+             "graph_assembled_ms": "one graph over the assembled matrices (sparse! cache)"}
+    best_mode = min(modes, key=lambda k: res[k])
+    best = res[best_mode]
     out = {
         "metric": "batched sector matvecs/s (29 sectors x 8 vectors, ComplexF32)",
         "value": len(items) * nb / (best * 1e-3),
@@ -242,6 +260,7 @@ def c4_workload(args, ev):
                      "unit": "GB/s", "frac": b_alg / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "bytes_alg_per_step": b_alg},
         "modes_ms": res,
+        "value_mode": modes[best_mode],
         "speedup_graph_vs_one_at_a_time": res["one_vector_at_a_time_ms"] / best,
         "autotuned_schedules": {VARIANT_NAMES[k]: v for k, v in tuned.items()},
         "basis_build_s": t_build,
