@@ -175,3 +175,35 @@ def test_l32_tile_schedule_matches_generic(gpu, kind, kw):
         yg = (H @ x).to_host()
         op.set_variant(dt, 1, _lib.EDK_VARIANT_AUTO)
         assert np.linalg.norm(yt - yg) <= tol * np.linalg.norm(yg), (kind, dt)
+
+
+@pytest.mark.parametrize("kind,L,kw,model", [
+    ("TranslationalBasis", 32, dict(N=16, k=3), xxz),              # complex phases, L = 32
+    ("TranslationParityBasis", 32, dict(N=16, k=16, p=-1), xxz),   # signs, L = 32
+    ("TranslationalBasis", 28, dict(N=14, k=5), j1j2),             # complex phases, L < 32
+    ("TranslationalBasis", 30, dict(N=15, k=15), xxz),             # k = pi signs, L < 32
+    ("TranslationParityBasis", 26, dict(N=13, k=13, p=-1), j1j2),  # signs, L < 32
+    ("TranslationParityBasis", 24, dict(N=12, k=0, p=1), j1j2),    # positive, L < 32
+])
+def test_run_length_paths_match_generic(gpu, kind, L, kw, model):
+    """The run-length canonicaliser (tile kernel for one vector, block kernel
+    for eight) against the generic kernel's full search, at sizes where a
+    large share of targets exercise both the direct and the deferred paths."""
+    import edkit_hip as edk
+    from edkit_hip import _lib
+    from edkit_hip.matvec import device_operator
+    B = getattr(edk, kind)(L=L, **kw)
+    H = model(B)
+    op = device_operator(H)
+    x1 = edk.DeviceArray.from_host(rand_state(B.D, 41))
+    op.set_variant(np.complex128, 1, _lib.EDK_VARIANT_GENERIC)
+    yg = (H @ x1).to_host()
+    op.set_variant(np.complex128, 1, _lib.EDK_VARIANT_AUTO)
+    yt = (H @ x1).to_host()
+    assert np.linalg.norm(yt - yg) <= 1e-13 * np.linalg.norm(yg), (kind, L, kw)
+    X = np.stack([rand_state(B.D, 50 + c) for c in range(8)], axis=1)
+    yb = (H @ edk.DeviceArray.from_host(X)).to_host()
+    op.set_variant(np.complex128, 8, _lib.EDK_VARIANT_GENERIC)
+    ybg = (H @ edk.DeviceArray.from_host(X)).to_host()
+    op.set_variant(np.complex128, 8, _lib.EDK_VARIANT_AUTO)
+    assert np.linalg.norm(yb - ybg) <= 1e-13 * np.linalg.norm(ybg), (kind, L, kw)
