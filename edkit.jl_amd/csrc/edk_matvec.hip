@@ -22,9 +22,11 @@
 #include "edk_matvec.h"
 
 // ---------------------------------------------------------------------------
-// generic kernel: one lane per row, any basis kind, any local terms
+// generic kernel: one lane per row, any basis kind, any local terms.  NV
+// columns [col, col + NV) of a row-interleaved block share each target's
+// canonical form and lookup.
 // ---------------------------------------------------------------------------
-template <typename VT, typename ST>
+template <typename VT, typename ST, int NV>
 __global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *__restrict__ x, VT *__restrict__ y,
                                                         int64_t nb, int64_t col, int accumulate) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -32,7 +34,9 @@ __global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *
   const TermsDev &T = A.T;
   uint64_t fm = (1ull << A.w) - 1;
   uint64_t s = A.kind == EDK_TENSOR ? edk_content_to_packed_d((uint64_t)i, A.L, A.base, A.w) : A.S.keys[i];
-  ST dr = 0, di = 0, orr = 0, oi = 0;
+  ST dr = 0, di = 0, orr[NV], oi[NV];
+#pragma unroll
+  for (int c = 0; c < NV; c++) orr[c] = oi[c] = 0;
   for (int t = 0; t < T.nterms; t++) {
     const int *td = T.tdesc + t * EDK_TDESC;
     int ns = td[0], m = td[1];
@@ -75,27 +79,55 @@ __global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *
         cr = o.phase.x * Rj;   // conj(phase) * R_j
         ci = -o.phase.y * Rj;
       }
-      VT xj = x[j * nb + col];
       // (v * coef) * xj
       ST ar = (ST)(v.x * cr - v.y * ci), ai = (ST)(v.x * ci + v.y * cr);
-      orr += ar * (ST)xj.x - ai * (ST)xj.y;
-      oi += ar * (ST)xj.y + ai * (ST)xj.x;
+      const VT *xj = x + j * nb + col;
+#pragma unroll
+      for (int c = 0; c < NV; c++) {
+        VT xv = xj[c];
+        orr[c] += ar * (ST)xv.x - ai * (ST)xv.y;
+        oi[c] += ar * (ST)xv.y + ai * (ST)xv.x;
+      }
     }
   }
-  VT xi = x[i * nb + col];
   ST invR = (A.kind == EDK_TENSOR || A.kind == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.R[i]);
-  ST yr = dr * (ST)xi.x - di * (ST)xi.y + orr * invR;
-  ST yi = dr * (ST)xi.y + di * (ST)xi.x + oi * invR;
-  VT out;
-  if (accumulate) {
-    VT y0 = y[i * nb + col];
-    out.x = y0.x + yr;
-    out.y = y0.y + yi;
-  } else {
-    out.x = yr;
-    out.y = yi;
+#pragma unroll
+  for (int c = 0; c < NV; c++) {
+    VT xi = x[i * nb + col + c];
+    ST yr = dr * (ST)xi.x - di * (ST)xi.y + orr[c] * invR;
+    ST yi = dr * (ST)xi.y + di * (ST)xi.x + oi[c] * invR;
+    VT out;
+    if (accumulate) {
+      VT y0 = y[i * nb + col + c];
+      out.x = y0.x + yr;
+      out.y = y0.y + yi;
+    } else {
+      out.x = yr;
+      out.y = yi;
+    }
+    y[i * nb + col + c] = out;
   }
-  y[i * nb + col] = out;
+}
+
+template <typename VT, typename ST>
+static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, int accumulate, hipStream_t st) {
+  unsigned grid = (unsigned)((a.D + 255) / 256);
+  for (int64_t c = 0; c < nb;) {
+    const int64_t left = nb - c;
+    if (left >= 8) {
+      k_matvec_generic<VT, ST, 8><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      c += 8;
+    } else if (left >= 4) {
+      k_matvec_generic<VT, ST, 4><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      c += 4;
+    } else if (left >= 2) {
+      k_matvec_generic<VT, ST, 2><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      c += 2;
+    } else {
+      k_matvec_generic<VT, ST, 1><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      c += 1;
+    }
+  }
 }
 
 // count of off-diagonal targets present in the sector (the NNZ_off figure)
@@ -314,16 +346,13 @@ extern "C" int edk_operator_count_offdiag(edk_op *op, int64_t *nnz) {
 int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st) {
   MatvecArgs a = op->args();
   if (a.D == 0) return EDK_OK;
-  unsigned grid = (unsigned)((a.D + 255) / 256);
   // several right-hand sides with the assembled matrix cached (sparse!)
   if (nb > 1 && edk_matvec_csr(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
-  for (int64_t c = 0; c < nb; c++) {
-    if (dtype == EDK_C64)
-      k_matvec_generic<double2, double><<<grid, 256, 0, st>>>(a, (const double2 *)x, (double2 *)y, nb, c, accumulate);
-    else
-      k_matvec_generic<float2, float><<<grid, 256, 0, st>>>(a, (const float2 *)x, (float2 *)y, nb, c, accumulate);
-  }
+  if (dtype == EDK_C64)
+    launch_generic<double2, double>(a, (const double2 *)x, (double2 *)y, nb, accumulate, st);
+  else
+    launch_generic<float2, float>(a, (const float2 *)x, (float2 *)y, nb, accumulate, st);
   EDK_HIP(hipGetLastError());
   return EDK_OK;
 }
