@@ -24,6 +24,7 @@ struct edk_csr {
   uint32_t *rowlen = nullptr;  // merged entries per row
   uint32_t *col = nullptr;     // 0-based columns
   double2 *val = nullptr;
+  float2 *valf = nullptr;      // the same values rounded to FP32 (ComplexF32 products)
 };
 
 // the target of local entry (t, c) of row state s: packed state and content delta
@@ -148,11 +149,17 @@ __global__ void k_asm_fill(MatvecArgs A, const uint64_t *rowptr, uint32_t *col, 
   rowlen[i] = k;
 }
 
+__global__ void k_asm_round_f32(const double2 *val, float2 *valf, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) valf[k] = make_float2((float)val[k].x, (float)val[k].y);
+}
+
 // Y = (accumulate ? Y : 0) + H X with NV consecutive lanes per row (one per
-// column), entries summed in column order
-template <typename VT, typename ST>
+// column), entries summed in column order.  HT is the stored value type:
+// FP64 for ComplexF64 vectors, FP32 for ComplexF32 vectors.
+template <typename VT, typename ST, typename HT>
 __global__ void __launch_bounds__(256) k_csr_spmm(const uint64_t *rowptr, const uint32_t *rowlen, const uint32_t *col,
-                                                  const double2 *val, int64_t D, int nv, const VT *__restrict__ x,
+                                                  const HT *val, int64_t D, int nv, const VT *__restrict__ x,
                                                   VT *__restrict__ y, int accumulate) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= D * nv) return;
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(256) k_csr_spmm(const uint64_t *rowptr, const 
   ST ar = 0, ai = 0;
   for (uint32_t e = 0; e < n; e++) {
     const uint32_t j = col[e0 + e];
-    const double2 h = val[e0 + e];
+    const HT h = val[e0 + e];
     const VT xj = x[(size_t)j * nv + c];
     ar += (ST)h.x * (ST)xj.x - (ST)h.y * (ST)xj.y;
     ai += (ST)h.x * (ST)xj.y + (ST)h.y * (ST)xj.x;
@@ -187,6 +194,7 @@ void edk_csr_release(edk_op *op) {
   (void)hipFree(M->rowlen);
   (void)hipFree(M->col);
   (void)hipFree(M->val);
+  (void)hipFree(M->valf);
   delete M;
   op->csr = nullptr;
 }
@@ -197,11 +205,11 @@ bool edk_matvec_csr(edk_op *op, int dtype, void *y, const void *x, int64_t nb, i
   const int64_t n = M->D * nb;
   const unsigned grid = (unsigned)((n + 255) / 256);
   if (dtype == EDK_C64)
-    k_csr_spmm<double2, double><<<grid, 256, 0, st>>>(M->rowptr, M->rowlen, M->col, M->val, M->D, (int)nb,
-                                                     (const double2 *)x, (double2 *)y, accumulate);
+    k_csr_spmm<double2, double, double2><<<grid, 256, 0, st>>>(M->rowptr, M->rowlen, M->col, M->val, M->D, (int)nb,
+                                                              (const double2 *)x, (double2 *)y, accumulate);
   else
-    k_csr_spmm<float2, float><<<grid, 256, 0, st>>>(M->rowptr, M->rowlen, M->col, M->val, M->D, (int)nb,
-                                                   (const float2 *)x, (float2 *)y, accumulate);
+    k_csr_spmm<float2, float, float2><<<grid, 256, 0, st>>>(M->rowptr, M->rowlen, M->col, M->valf, M->D, (int)nb,
+                                                           (const float2 *)x, (float2 *)y, accumulate);
   return hipGetLastError() == hipSuccess;
 }
 
@@ -237,13 +245,16 @@ extern "C" int edk_operator_assemble(edk_op *op, int64_t *nnz) {
   if (!rc && (hipMalloc((void **)&M->rowptr, (a.D + 1) * sizeof(uint64_t)) != hipSuccess ||
               hipMalloc((void **)&M->rowlen, std::max<int64_t>(a.D, 1) * sizeof(uint32_t)) != hipSuccess ||
               hipMalloc((void **)&M->col, std::max<int64_t>(M->slots, 1) * sizeof(uint32_t)) != hipSuccess ||
-              hipMalloc((void **)&M->val, std::max<int64_t>(M->slots, 1) * sizeof(double2)) != hipSuccess))
+              hipMalloc((void **)&M->val, std::max<int64_t>(M->slots, 1) * sizeof(double2)) != hipSuccess ||
+              hipMalloc((void **)&M->valf, std::max<int64_t>(M->slots, 1) * sizeof(float2)) != hipSuccess))
     rc = edk_fail(EDK_ERR_OOM, "sector matrix does not fit in device memory");
   if (!rc && hipMemcpyAsync(M->rowptr, rp.data(), (a.D + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
                             ctx->stream) != hipSuccess)
     rc = edk_fail(EDK_ERR_HIP, "row offsets");
   if (!rc && a.D > 0) {
     k_asm_fill<<<grid, 256, 0, ctx->stream>>>(a, M->rowptr, M->col, M->val, M->rowlen);
+    if (M->slots > 0)
+      k_asm_round_f32<<<(unsigned)((M->slots + 255) / 256), 256, 0, ctx->stream>>>(M->val, M->valf, M->slots);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(cnt.data(), M->rowlen, a.D * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream) !=
             hipSuccess ||
