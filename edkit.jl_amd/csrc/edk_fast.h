@@ -116,10 +116,15 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
   }
   const int sh = A.A;
   int period = A.len;
-  for (int q = 0; q < A.ndiv; q++) {
-    int p = A.divs[q];
-    bool m = rotr32(s, p * sh, A.L, A.mask) == s;
-    period = (m && period == A.len) ? p : period;
+  // constant indices into the argument struct: a run-time index would make
+  // the compiler copy the whole struct to per-lane scratch at kernel entry
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    if (q < A.ndiv) {
+      const int p = A.divs[q];
+      const bool m = rotr32(s, p * sh, A.L, A.mask) == s;
+      period = (m && period == A.len) ? p : period;
+    }
   }
   if (KIND == EDK_TRANSLATIONAL) {
     uint32_t minA = s, cur = s;
