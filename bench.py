@@ -297,7 +297,9 @@ def c5_stream(n, rate, seed):
 
 def c5_workload(args, ev):
     """Serving stream through the server (continuous admission, `workers`
-    streams) versus the same jobs solved one at a time."""
+    streams) versus the same jobs solved one at a time.  Sectors and
+    operators are built before timing; the one-at-a-time pass runs first
+    (fresh memory pool), then the served pass."""
     import edkit_hip as edk
     sys.path.insert(0, ROOT)
     from tests.models import blbq, j1j2, xxz
@@ -322,20 +324,37 @@ def c5_workload(args, ev):
     def submit(srv, j, H):
         idx = min(int(j["start"] * H.B.D), H.B.D - 1)
         if j["kind"] == "ground":
-            return srv.submit_ground(H, psi0_index=idx, m=300, tol=1e-9)
+            return srv.submit_ground(H, psi0_index=idx, m=300, tol=1e-9, reorth=args.reorth)
         return srv.submit_evolve(H, j["t"], psi0_index=idx)
 
+    # every sector and operator is built (and its matvec tables set up by one
+    # product) before either timed phase, so both phases time the solves only
+    t_build = time.perf_counter()
+    ops = [operator_of(j) for j in jobs]
+    for H in {id(H): H for H in ops}.values():
+        _ = H @ edk.DeviceArray((H.B.D,), np.complex128)
+    for B in bases.values():
+        B.ctx.synchronize()
+    t_build = time.perf_counter() - t_build
+    # one at a time first (same jobs and operators), then served
+    seq_res = []
+    with edk.Server(workers=1) as srv:
+        s0 = srv.clock()
+        for i, (j, H) in enumerate(zip(jobs, ops)):
+            r, _ = srv.wait(submit(srv, j, H))
+            seq_res.append(r)
+            if (i + 1) % 50 == 0:
+                print(f"[c5] one-at-a-time {i + 1}/{len(jobs)}", file=sys.stderr, flush=True)
+        seq_span = srv.clock() - s0
+    seq = len(jobs) / seq_span
     # served: requests submitted at their arrival times
-    ops = []
     with edk.Server(workers=args.workers) as srv:
         t0 = srv.clock()
         tickets = []
-        for j in jobs:
+        for j, H in zip(jobs, ops):
             now = srv.clock() - t0
             if j["arrival"] > now:
                 time.sleep(j["arrival"] - now)
-            H = operator_of(j)
-            ops.append(H)
             tickets.append(submit(srv, j, H))
         res = []
         for i, tk in enumerate(tickets):
@@ -348,17 +367,6 @@ def c5_workload(args, ev):
     lat = np.array([r["t_end"] - t0 - j["arrival"] for r, j in zip(res, jobs)])
     span = max(r["t_end"] for r in res) - t0
     served = len(jobs) / span
-    # one at a time, same jobs and operators
-    seq_res = []
-    with edk.Server(workers=1) as srv:
-        s0 = srv.clock()
-        for i, (j, H) in enumerate(zip(jobs, ops)):
-            r, _ = srv.wait(submit(srv, j, H))
-            seq_res.append(r)
-            if (i + 1) % 50 == 0:
-                print(f"[c5] one-at-a-time {i + 1}/{len(jobs)}", file=sys.stderr, flush=True)
-        seq_span = srv.clock() - s0
-    seq = len(jobs) / seq_span
     same = all(a["E0"] == b["E0"] and a["overlap_re"] == b["overlap_re"] and a["overlap_im"] == b["overlap_im"]
                for a, b in zip(res, seq_res))
     ground = [r for r, j in zip(res, jobs) if j["kind"] == "ground"]
@@ -366,12 +374,14 @@ def c5_workload(args, ev):
     # mean matvecs and share of the total
     classes = {}
     for r, j in zip(seq_res, jobs):
-        c = classes.setdefault(f"L{j['L']}{'s1' if j['base'] == 3 else ''}-{j['kind']}", [0, 0.0, 0])
+        c = classes.setdefault(f"L{j['L']}{'s1' if j['base'] == 3 else ''}-{j['kind']}", [0, 0.0, 0, []])
         c[0] += 1
         c[1] += r["t_end"] - r["t_start"]
         c[2] += r["matvecs"]
+        c[3].append(r["t_end"] - r["t_start"])
     tot = sum(c[1] for c in classes.values())
-    breakdown = {k: {"n": c[0], "mean_s": c[1] / c[0], "mean_matvecs": c[2] / c[0], "share": c[1] / tot}
+    breakdown = {k: {"n": c[0], "mean_s": c[1] / c[0], "median_s": float(np.median(c[3])),
+                     "mean_matvecs": c[2] / c[0], "share": c[1] / tot}
                  for k, c in sorted(classes.items(), key=lambda kv: -kv[1][1])}
     out = {
         "metric": "mixed-solve req/s (served) with p50/p99 latency",
@@ -381,7 +391,7 @@ def c5_workload(args, ev):
         "config": {"workload": f"serving stream: {len(jobs)} requests, Poisson {args.rate} req/s, spin-1/2 "
                                "L=18..30 XXZ/J1-J2 and spin-1 BLBQ L=12..16, random k/p sectors, Sz=0; "
                                "60% Lanczos to residual 1e-9, 40% Krylov evolution to t in [1,20]",
-                   "workers": args.workers, "parallelism": "single GPU"},
+                   "workers": args.workers, "ground_reorth": args.reorth, "parallelism": "single GPU"},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "mean": float(lat.mean())},
         "one_at_a_time_req_s": seq,
@@ -390,7 +400,7 @@ def c5_workload(args, ev):
         "failed": len(bad), "first_error": bad[0] if bad else None,
         "max_ground_resid": max((r["resid"] for r in ground), default=None),
         "peak_concurrent_jobs": st["peak_running"], "peak_inflight_GB": st["peak_inflight_bytes"] / 1e9,
-        "sectors_built": len(bases),
+        "sectors_built": len(bases), "sector_build_s": t_build,
         "one_at_a_time_breakdown": breakdown,
     }
     print(json.dumps(out))
@@ -530,6 +540,8 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="c5: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate, req/s")
     ap.add_argument("--workers", type=int, default=8, help="c5: server worker streams")
+    ap.add_argument("--reorth", default="partial", choices=["full", "partial"],
+                    help="c5: reorthogonalisation of the ground-state jobs")
     ap.add_argument("--lanczos-steps", type=int, default=300,
                     help="steps of the ground-state Lanczos leg (0 skips it)")
     args = ap.parse_args()

@@ -26,18 +26,42 @@ struct KrylovWork {
   int scale(void *y, const void *x, double s);
   int cgs_sweep(const void *V, int64_t ldv, int j, void *w);
   // reorthogonalise w against V[:, :j] (one or two sweeps) and return ||w||
-  int reorth_norm(const void *V, int64_t ldv, int j, void *w, double *wnorm, int *sweeps);
+  int reorth_norm(const void *V, int64_t ldv, int j, void *w, double *wnorm, int *sweeps, double *hmax = nullptr);
   int combine(const void *V, int64_t ldv, int j, const double2 *c, double scale, void *out, int out_dtype);
 };
 
 int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, double *Z, int nr,
                     const int *rows);
+// Partial reorthogonalisation (H. D. Simon, "The Lanczos algorithm with
+// partial reorthogonalization", Math. Comp. 42 (1984) 115-142): the estimates
+// omega[j][k] ~ v_j^H v_k follow the Lanczos recurrence itself,
+//   beta_j w_{j+1,k} = beta_k w_{j,k+1} + (alpha_k - alpha_j) w_{j,k}
+//                      + beta_{k-1} w_{j,k-1} - beta_{j-1} w_{j-1,k} +- theta,
+// with theta = 10 eps ||T_j|| the rounding term (||T_j|| by Gershgorin).  When
+// an estimate passes sqrt(eps), the new vector and the next one are
+// reorthogonalised against the whole basis and their estimates reset to eps;
+// the basis stays semi-orthogonal, which keeps Ritz values and the residual
+// estimate accurate to working precision (Simon, section 5).
+struct PartialOrth {
+  double eps = 0, anorm = 0;
+  std::vector<double> wold, wcur;  // rows j-1 and j of omega (k = 1..)
+  bool force = false, forced_now = false;
+  int reorths = 0;
+  double hmax = 0;                 // largest |v_k^H w| / ||w|| met at a reorthogonalisation
+  explicit PartialOrth(double e) : eps(e), wcur(1, 1.0) {}
+  // omega row j+1 from the provisional beta_j; true: reorthogonalise v_{j+1}
+  bool update(int j, const double *alpha, const double *beta, double bj);
+  // after v_{j+1} has been reorthogonalised against v_1..v_j
+  void reset(int j);
+};
+
 int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta, int *m,
-                   int m_target, int64_t *matvecs, int ncols);
+                   int m_target, int64_t *matvecs, int ncols, PartialOrth *po = nullptr);
 
 // the Lanczos ground state and the Krylov cache on a given context (its stream
 // and allocation mode), for callers that run several at once
 int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, double tol, int refine, double *E0,
-                    double *resid, int *steps, void *vec_out);
+                    double *resid, int *steps, void *vec_out, int reorth = EDK_REORTH_FULL,
+                    int *reorth_steps = nullptr);
 int edk_krylov_create_on(edk_op *op, edk_ctx *ctx, int dtype, const void *psi0, const edk_krylov_params *p,
                          edk_krylov **out);

@@ -279,7 +279,7 @@ int KrylovWork::cgs_sweep(const void *V, int64_t ldv, int j, void *w) {
 // Lanczos step w is already nearly orthogonal to V, so the second sweep --
 // and its two passes over V -- is rarely needed; when it is, two sweeps
 // leave w orthogonal to working precision ("twice is enough").
-int KrylovWork::reorth_norm(const void *V, int64_t ldv, int j, void *w, double *wnorm, int *sweeps) {
+int KrylovWork::reorth_norm(const void *V, int64_t ldv, int j, void *w, double *wnorm, int *sweeps, double *hmax) {
   hipStream_t s = ctx->stream;
   if (int rc = cgs_sweep(V, ldv, j, w)) return rc;
   // h back to the host while the norm of w' is reduced (one synchronisation)
@@ -289,9 +289,14 @@ int KrylovWork::reorth_norm(const void *V, int64_t ldv, int j, void *w, double *
   EDK_HIP(hipGetLastError());
   EDK_HIP(hipMemcpyAsync(h_s + NORM_SLOT, d_s + NORM_SLOT, sizeof(double2), hipMemcpyDeviceToHost, s));
   EDK_HIP(hipStreamSynchronize(s));
-  double hh = 0.0;
-  for (int c = 0; c < j; c++) hh += h_s[c].x * h_s[c].x + h_s[c].y * h_s[c].y;
+  double hh = 0.0, hm = 0.0;
+  for (int c = 0; c < j; c++) {
+    const double a = h_s[c].x * h_s[c].x + h_s[c].y * h_s[c].y;
+    hh += a;
+    hm = std::max(hm, a);
+  }
   double w2 = h_s[NORM_SLOT].x;
+  if (hmax) *hmax = std::sqrt(hm / std::max(w2 + hh, 1e-300));
   int nsw = 1;
   if (w2 < 0.5 * (w2 + hh)) {
     if (int rc = cgs_sweep(V, ldv, j, w)) return rc;
@@ -397,10 +402,43 @@ int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, d
 // ---------------------------------------------------------------------------
 // Lanczos step: the _lanczos_build! recurrence (TimeEvolution.jl:250-306)
 // ---------------------------------------------------------------------------
+bool PartialOrth::update(int j, const double *alpha, const double *beta, double bj) {
+  const double aj = alpha[j - 1], bjm1 = j > 1 ? beta[j - 2] : 0.0;
+  anorm = std::max(anorm, std::fabs(aj) + bjm1 + bj);
+  const double th = 10.0 * eps * anorm;
+  std::vector<double> wn(j + 1);
+  double mx = 0.0;
+  if (bj > 0) {
+    for (int k = 1; k <= j - 1; k++) {
+      double t = beta[k - 1] * wcur[k] + (alpha[k - 1] - aj) * wcur[k - 1] - bjm1 * wold[k - 1];
+      if (k > 1) t += beta[k - 2] * wcur[k - 2];
+      t = (t + std::copysign(th, t)) / bj;
+      wn[k - 1] = t;
+      mx = std::max(mx, std::fabs(t));
+    }
+    wn[j - 1] = th / bj;  // consecutive vectors: the local alpha/beta removal
+  } else {
+    mx = INFINITY;
+  }
+  wn[j] = 1.0;
+  wold.swap(wcur);
+  wcur.swap(wn);
+  forced_now = force;
+  return force || !(mx <= std::sqrt(eps));
+}
+
+void PartialOrth::reset(int j) {
+  for (int k = 0; k < j; k++) wcur[k] = eps;
+  force = !forced_now;  // a threshold crossing also reorthogonalises the next vector
+  reorths++;
+}
+
 // Extends the basis V (columns 0..) from m_start to at most m_target steps.
 // Returns 1 on lucky breakdown (beta set to 0), 0 otherwise; <0 on error.
+// po == nullptr: full reorthogonalisation at every step (the reference's
+// two-sweep Gram-Schmidt); otherwise partial reorthogonalisation.
 int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta,
-                   int *m, int m_target, int64_t *matvecs, int ncols) {
+                   int *m, int m_target, int64_t *matvecs, int ncols, PartialOrth *po) {
   size_t es = K.esize();
   for (int j = *m + 1; j <= m_target; j++) {
     char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
@@ -416,7 +454,17 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
     alpha[j - 1] = aj;
     if (int rc = K.axpy(w, vj, make_double2(-aj, 0.0))) return -rc;
     double bj;
-    if (int rc = K.reorth_norm(V, ldv, j, w, &bj, nullptr)) return -rc;
+    if (!po) {
+      if (int rc = K.reorth_norm(V, ldv, j, w, &bj, nullptr)) return -rc;
+    } else {
+      if (int rc = K.norm(w, &bj)) return -rc;
+      if (po->update(j, alpha, beta, bj)) {
+        double hm = 0;
+        if (int rc = K.reorth_norm(V, ldv, j, w, &bj, nullptr, &hm)) return -rc;
+        po->hmax = std::max(po->hmax, hm);
+        po->reset(j);
+      }
+    }
     beta[j - 1] = bj;
     const double sq = 1.4901161193847656e-08;  // sqrt(eps(Float64))
     if (bj <= sq * std::max(1.0, std::fabs(aj))) {
@@ -437,9 +485,11 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
 // Lanczos ground state
 // ---------------------------------------------------------------------------
 int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, double tol, int refine, double *E0,
-                    double *resid, int *steps, void *vec_out) {
+                    double *resid, int *steps, void *vec_out, int reorth, int *reorth_steps) {
   if (!op || !ctx || !x0 || !E0 || !resid || m < 1) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground: bad argument");
   if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "bad dtype");
+  if (reorth != EDK_REORTH_FULL && reorth != EDK_REORTH_PARTIAL)
+    return edk_fail(EDK_ERR_ARG, "reorth must be EDK_REORTH_FULL or EDK_REORTH_PARTIAL");
   if (m > EDK_KRYLOV_MAXM) return edk_fail(EDK_ERR_UNSUPPORTED, "Lanczos dimension above 1024");
   int64_t D = op->basis->D;
   if (D < 1) return edk_fail(EDK_ERR_ARG, "empty sector");
@@ -466,8 +516,11 @@ int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, 
   if (!rc && !(nrm > 0)) rc = edk_fail(EDK_ERR_NUMERIC, "Initial state has zero norm");
   if (!rc) rc = K.scale(V, x0, 1.0 / nrm);
   double e0 = 0, res = 0;
+  // unit roundoff of the stored vectors
+  PartialOrth po(dtype == EDK_C64 ? 2.220446049250313e-16 : 1.1920928955078125e-07);
+  PartialOrth *pop = reorth == EDK_REORTH_PARTIAL ? &po : nullptr;
   for (int j = 1; !rc && j <= m; j++) {
-    int br = lanczos_extend(op, K, V, D, w, alpha.data(), beta.data(), &mm, j, &mv, m + 1);
+    int br = lanczos_extend(op, K, V, D, w, alpha.data(), beta.data(), &mm, j, &mv, m + 1, pop);
     if (br < 0) {
       rc = -br;
       break;
@@ -527,6 +580,7 @@ int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, 
   *E0 = e0;
   *resid = res;
   if (steps) *steps = mm;
+  if (reorth_steps) *reorth_steps = pop ? po.reorths : mm;
   return EDK_OK;
 }
 
@@ -534,6 +588,12 @@ extern "C" int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m, 
                                   double *resid, int *steps, void *vec_out) {
   if (!op) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground: null operator");
   return edk_lanczos_run(op, op->ctx, dtype, x0, m, tol, refine, E0, resid, steps, vec_out);
+}
+
+extern "C" int edk_lanczos_ground_ex(edk_op *op, int dtype, const void *x0, int m, double tol, int refine, int reorth,
+                                     double *E0, double *resid, int *steps, int *reorth_steps, void *vec_out) {
+  if (!op) return edk_fail(EDK_ERR_ARG, "edk_lanczos_ground_ex: null operator");
+  return edk_lanczos_run(op, op->ctx, dtype, x0, m, tol, refine, E0, resid, steps, vec_out, reorth, reorth_steps);
 }
 
 // ---------------------------------------------------------------------------

@@ -97,7 +97,7 @@ static void run_job(edk_server *S, edk_ctx *ctx, ServerJob *J) {
   if (!rc && j.kind == EDK_JOB_GROUND) {
     int steps = 0;
     rc = edk_lanczos_run(j.op, ctx, j.dtype, x0, j.m, j.tol, j.refine, &r.E0, &r.resid, &steps,
-                         j.want_vector ? out : nullptr);
+                         j.want_vector ? out : nullptr, j.reorth);
     r.steps = steps;
     r.matvecs = steps + (j.refine ? 1 : 0);
   } else if (!rc) {
@@ -232,6 +232,8 @@ extern "C" int edk_server_submit(edk_server *S, const edk_job *job, int64_t *tic
   if (j.kind == EDK_JOB_GROUND && (j.m < 1 || j.m > EDK_KRYLOV_MAXM))
     return edk_fail(EDK_ERR_ARG, "Lanczos steps must be 1..1024");
   if (j.kind == EDK_JOB_EVOLVE && !(j.t >= 0)) return edk_fail(EDK_ERR_ARG, "evolution time must be >= 0");
+  if (j.kind == EDK_JOB_GROUND && j.reorth != EDK_REORTH_FULL && j.reorth != EDK_REORTH_PARTIAL)
+    return edk_fail(EDK_ERR_ARG, "reorth must be EDK_REORTH_FULL or EDK_REORTH_PARTIAL");
   // the kernels' one-time setup runs here, in the caller's thread
   EDK_HIP(hipSetDevice(S->device));
   (void)edk_matvec_prepare(j.op);

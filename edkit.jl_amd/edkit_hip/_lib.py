@@ -33,6 +33,7 @@ SYMBOLS = [
     "edk_server_stats", "edk_server_destroy",
     "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2",
     "edk_lanczos_ground",
+    "edk_lanczos_ground_ex",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
     "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
 ]
@@ -60,13 +61,15 @@ class BatchItem(C.Structure):
 
 
 EDK_JOB_GROUND, EDK_JOB_EVOLVE = 0, 1
+EDK_REORTH_FULL, EDK_REORTH_PARTIAL = 0, 1
+REORTH_CODES = {"full": EDK_REORTH_FULL, "partial": EDK_REORTH_PARTIAL}
 
 
 class Job(C.Structure):
     _fields_ = [("kind", C.c_int32), ("dtype", C.c_int32), ("op", C.c_void_p), ("psi0", C.c_void_p),
                 ("psi0_index", C.c_int64), ("m", C.c_int32), ("refine", C.c_int32), ("tol", C.c_double),
                 ("t", C.c_double), ("m_init", C.c_int32), ("m_max", C.c_int32),
-                ("want_vector", C.c_int32)]
+                ("want_vector", C.c_int32), ("reorth", C.c_int32)]
 
 
 class JobResult(C.Structure):
@@ -154,6 +157,8 @@ def lib():
         "edk_vec_nrm2": (I32, [P, I32, P, I64, C.POINTER(D)]),
         "edk_lanczos_ground": (I32, [P, I32, P, I32, D, I32, C.POINTER(D), C.POINTER(D),
                                      C.POINTER(C.c_int), P]),
+        "edk_lanczos_ground_ex": (I32, [P, I32, P, I32, D, I32, I32, C.POINTER(D), C.POINTER(D),
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), P]),
         "edk_krylov_default_params": (None, [C.POINTER(KrylovParams)]),
         "edk_krylov_create": (I32, [P, I32, P, C.POINTER(KrylovParams), PP]),
         "edk_krylov_evolve": (I32, [P, D, P]),

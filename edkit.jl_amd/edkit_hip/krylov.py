@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import KrylovDiag, KrylovParams, check, lib
+from ._lib import REORTH_CODES, KrylovDiag, KrylovParams, check, lib
 from .device import DeviceArray, dev_ptr, dtype_code
 from .matvec import device_operator
 
@@ -19,20 +19,29 @@ def _to_device(v, B, dtype):
     return v
 
 
-def lanczos_ground(H, v0, m=300, tol=0.0, dtype=np.complex128, refine=True, want_vec=False):
-    """Lowest eigenpair by Lanczos with full reorthogonalisation.
-    Returns (E0, residual, steps, vec or None).  dtype complex64 stores the
-    Krylov vectors and runs the matvec in FP32 with FP64 scalars; refine adds
-    the final FP64 Rayleigh quotient/residual on the Ritz vector."""
+def lanczos_ground(H, v0, m=300, tol=0.0, dtype=np.complex128, refine=True, want_vec=False,
+                   reorth="full", info=None):
+    """Lowest eigenpair by Lanczos.  Returns (E0, residual, steps, vec or None).
+    dtype complex64 stores the Krylov vectors and runs the matvec in FP32
+    with FP64 scalars; refine adds the final FP64 Rayleigh quotient/residual
+    on the Ritz vector.  reorth: "full" reorthogonalises every step against
+    the whole basis; "partial" only when the omega-recurrence estimate of the
+    loss of orthogonality passes sqrt(eps) (Simon 1984).  info (a dict, if
+    given) receives "reorth_steps"."""
+    if reorth not in REORTH_CODES:
+        raise ValueError("reorth must be 'full' or 'partial'")
     op = device_operator(H)
     B = H.B
     x0 = _to_device(v0, B, dtype)
     dt = x0.dtype
     vec = DeviceArray((B.D,), dt, ctx=B.ctx, zero=False) if want_vec else None
-    E0, res, steps = C.c_double(), C.c_double(), C.c_int()
-    check(lib().edk_lanczos_ground(op.h, dtype_code(dt), dev_ptr(x0), int(m), float(tol),
-                                   int(refine), C.byref(E0), C.byref(res), C.byref(steps),
-                                   vec.ptr if vec is not None else None))
+    E0, res, steps, nre = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+    check(lib().edk_lanczos_ground_ex(op.h, dtype_code(dt), dev_ptr(x0), int(m), float(tol),
+                                      int(refine), REORTH_CODES[reorth], C.byref(E0), C.byref(res),
+                                      C.byref(steps), C.byref(nre),
+                                      vec.ptr if vec is not None else None))
+    if info is not None:
+        info["reorth_steps"] = nre.value
     return E0.value, res.value, steps.value, vec
 
 

@@ -7,7 +7,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (EDK_JOB_EVOLVE, EDK_JOB_GROUND, Job, JobResult, ServerInfo, check, lib)
+from ._lib import (EDK_JOB_EVOLVE, EDK_JOB_GROUND, REORTH_CODES, Job, JobResult, ServerInfo, check,
+                   lib)
 from .device import dtype_code
 from .matvec import device_operator
 
@@ -34,7 +35,8 @@ class Server:
             ptr = keep.ctypes.data
         job = Job(kind, dtype_code(dt), op.h.value, ptr, int(psi0_index), int(kw.get("m", 0)),
                   int(kw.get("refine", 0)), float(kw.get("tol", 0.0)), float(kw.get("t", 0.0)),
-                  int(kw.get("m_init", 0)), int(kw.get("m_max", 0)), int(kw.get("want_vector", 0)))
+                  int(kw.get("m_init", 0)), int(kw.get("m_max", 0)), int(kw.get("want_vector", 0)),
+                  REORTH_CODES[kw.get("reorth", "full")])
         tk = C.c_int64()
         check(lib().edk_server_submit(self.h, C.byref(job), C.byref(tk)))
         # the operator must outlive the job; the start vector was copied
@@ -42,11 +44,14 @@ class Server:
         return tk.value
 
     def submit_ground(self, H, psi0=None, psi0_index=0, m=300, tol=1e-9, refine=False,
-                      dtype=np.complex128, want_vector=False):
+                      dtype=np.complex128, want_vector=False, reorth="full"):
         """Lanczos ground state of H from psi0 (host vector) or the product
-        state at psi0_index; returns a ticket."""
+        state at psi0_index; returns a ticket.  reorth: "full" or "partial"
+        (as in lanczos_ground)."""
+        if reorth not in REORTH_CODES:
+            raise ValueError("reorth must be 'full' or 'partial'")
         return self._submit(H, EDK_JOB_GROUND, dtype, psi0, psi0_index, m=m, tol=tol,
-                            refine=int(refine), want_vector=int(want_vector))
+                            refine=int(refine), want_vector=int(want_vector), reorth=reorth)
 
     def submit_evolve(self, H, t, psi0=None, psi0_index=0, tol=1e-10, m_init=30, m_max=60,
                       dtype=np.complex128, want_vector=False):

@@ -227,6 +227,17 @@ int edk_vec_nrm2(edk_ctx *ctx, int dtype, const void *x, int64_t n, double *out)
 int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m,
                        double tol, int refine, double *E0, double *resid,
                        int *steps, void *vec_out);
+#define EDK_REORTH_FULL 0    /* every step: against all previous vectors (DGKS) */
+#define EDK_REORTH_PARTIAL 1 /* Simon's partial reorthogonalisation: only when the
+                                omega-recurrence estimate of the loss of
+                                orthogonality passes sqrt(eps) (then at that step
+                                and the next); the basis stays semi-orthogonal */
+/* edk_lanczos_ground with a choice of reorthogonalisation (no counterpart in
+ * the reference).  reorth_steps (optional): steps that reorthogonalised. */
+int edk_lanczos_ground_ex(edk_op *op, int dtype, const void *x0, int m,
+                          double tol, int refine, int reorth, double *E0,
+                          double *resid, int *steps, int *reorth_steps,
+                          void *vec_out);
 
 /* KrylovEvolutionCache (TimeEvolution.jl:84-218), device-resident. */
 typedef struct {
@@ -283,6 +294,7 @@ typedef struct {
   int32_t m_init;      /* evolve: Krylov dimensions (<= 0: 30 and 60) */
   int32_t m_max;
   int32_t want_vector; /* 1: keep the result vector (ground state or psi(t)) */
+  int32_t reorth;      /* ground: EDK_REORTH_FULL or EDK_REORTH_PARTIAL */
 } edk_job;
 typedef struct {
   int32_t status;      /* EDK_OK or the error code of the job */

@@ -209,6 +209,42 @@ def test_lanczos_matches_oracle(gpu):
     assert abs(E32 - E_o) < 1e-9
 
 
+@pytest.mark.parametrize("kind,L,kw,model,dt", [
+    ("TranslationalBasis", 20, dict(N=10, k=3), j1j2, np.complex128),
+    ("TranslationParityBasis", 22, dict(N=11, k=0, p=1), xxz, np.complex128),
+    ("TranslationalBasis", 18, dict(N=9, k=0), xxz, np.complex64),
+])
+def test_partial_reorth_matches_full(gpu, kind, L, kw, model, dt):
+    """Partial reorthogonalisation (omega recurrence) against full
+    reorthogonalisation and the oracle's Lanczos: the same energy to working
+    precision, a true residual at the requested level, the same Ritz vector,
+    and far fewer reorthogonalisation steps."""
+    import edkit_hip as edk
+    B, ob = make_pair(edk, kind, L, **kw)
+    H = model(B)
+    v0 = rand_state(B.D, 8)
+    E_o, _, _, _ = orc.lanczos_ground(orc.Op(ob, H.flatten()), v0, 300, 1e-10)
+    out = {}
+    for mode in ("full", "partial"):
+        info = {}
+        E, res, steps, vec = edk.lanczos_ground(H, v0.astype(dt), m=300, tol=1e-10, dtype=dt, refine=True,
+                                                want_vec=True, reorth=mode, info=info)
+        out[mode] = (E, res, steps, vec.to_host().astype(np.complex128), info["reorth_steps"])
+    Ef, rf, sf, yf, nf = out["full"]
+    Ep, rp, sp, yp, npart = out["partial"]
+    tolE = 1e-11 if dt == np.complex128 else 1e-9
+    assert abs(Ef - E_o) < tolE and abs(Ep - E_o) < tolE, (Ef, Ep, E_o)
+    assert nf == sf
+    assert npart < sp // 3, (npart, sp)
+    if dt == np.complex128:
+        assert abs(sp - sf) <= 2
+        # refined (true) residuals of the normalised Ritz vectors
+        assert rf < 1e-9 and rp < 1e-9, (rf, rp)
+        r = H @ yp - Ep * yp
+        assert np.linalg.norm(r) < 1e-9 * np.linalg.norm(yp)
+        assert abs(abs(np.vdot(yf, yp)) / (np.linalg.norm(yf) * np.linalg.norm(yp)) - 1) < 1e-12
+
+
 def test_krylov_matches_oracle(gpu):
     import edkit_hip as edk
     B, ob = make_pair(edk, "TranslationalBasis", 14, N=7, k=0)

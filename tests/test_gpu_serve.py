@@ -30,6 +30,7 @@ def job_mix(edk):
     B6 = edk.TranslationalBasis(L=18, N=9, k=5)
     jobs.append(("ground", xxz(B6, 0.9), dict(psi0=rand_state(B6.D, 16), m=120, tol=1e-9)))
     jobs.append(("evolve", j1j2(B6, 0.5), dict(psi0=rand_state(B6.D, 17), t=1.0, dtype=np.complex64)))
+    jobs.append(("ground", j1j2(B6, 0.3), dict(psi0=rand_state(B6.D, 18), m=200, tol=1e-9, reorth="partial")))
     return jobs
 
 
@@ -62,7 +63,8 @@ def test_results_same_alone_and_in_mix(gpu):
         if kind != "ground":
             continue
         E, r, steps, vec = edk.lanczos_ground(H, kw["psi0"], m=kw["m"], tol=kw["tol"],
-                                              refine=kw.get("refine", False), want_vec=True)
+                                              refine=kw.get("refine", False), want_vec=True,
+                                              reorth=kw.get("reorth", "full"))
         assert (E, r, steps) == (ra["E0"], ra["resid"], ra["steps"])
         assert np.array_equal(vec.to_host(), va)
 
