@@ -96,6 +96,18 @@ __device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t m
   uint32_t hi = s >> (32 - L), lo = s | (s << L);
   return __builtin_amdgcn_alignbit(hi, lo, sh) & mask;
 }
+// the same for a length known to be below 32 (no branch on L)
+__device__ __forceinline__ uint32_t rotr_short(uint32_t s, int sh, int L, uint32_t mask) {
+  return __builtin_amdgcn_alignbit(s >> (32 - L), s | (s << L), sh) & mask;
+}
+// ring rotation for a kernel flavour: LT == 32 one v_alignbit, LT == 1
+// (16 <= L < 32) a funnel without a branch, otherwise either
+template <int LT>
+__device__ __forceinline__ uint32_t rotr_lt(uint32_t s, int sh, int L, uint32_t mask) {
+  if constexpr (LT == 32) return __builtin_amdgcn_alignbit(s, s, sh);
+  else if constexpr (LT == 1) return rotr_short(s, sh, L, mask);
+  else return rotr32(s, sh, L, mask);
+}
 
 __device__ __forceinline__ uint32_t rev32(uint32_t s, int L) { return __brev(s) >> (32 - L); }
 
@@ -329,8 +341,103 @@ __device__ __forceinline__ uint32_t rotl_w(uint32_t x, uint32_t m, int L, uint32
   if (LT == 32) return __builtin_amdgcn_alignbit(x, x, (32u - m) & 31u);
   return rotr32(x, m == 0 ? 0 : L - (int)m, L, mask);  // 0 <= m < L
 }
+// The run-length canonicaliser for 16 <= L < 32 on periodic windows: an
+// L-bit ring word x is held top-aligned in 32 bits with its low c = 32 - L
+// bits continuing the ring (W = x << c | x >> (L - c)), so the window is the
+// first 32 bits of the infinite string x x x ...  Then a rotation is one
+// v_alignbit of (W, W << c):
+//   rotl_L(x, m) -> alignbit(W, W << c, 32 - m)   for 1 <= m <= L,
+//   rotr_L(x, m) -> alignbit(W, W << c, c + m)    for 0 <= m <  L,
+// windows order as their words do, and bitwise operations keep windows
+// periodic.  Same candidates, rotation counts and results as canon_runs.
+template <int KIND, int PH>
+__device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci) {
+  const uint32_t L = (uint32_t)B.L, c = 32u - L;
+  const uint32_t W = (s << c) | (s >> (L - c)), Wc = W << c;
+#define EDK_WROTL(X, XC, M) __builtin_amdgcn_alignbit((X), (XC), 32u - (M))
+#define EDK_WROTR(X, XC, M) __builtin_amdgcn_alignbit((X), (XC), c + (M))
+  // longest zero runs: binary lifting z_2m = z_m & rotl(z_m, m)
+  const uint32_t z1 = ~W, z1c = z1 << c;
+  const uint32_t z2 = z1 & EDK_WROTL(z1, z1c, 1u), z2c = z2 << c;
+  const uint32_t z4 = z2 & EDK_WROTL(z2, z2c, 2u), z4c = z4 << c;
+  const uint32_t z8 = z4 & EDK_WROTL(z4, z4c, 4u), z8c = z8 << c;
+  const uint32_t z16 = z8 & EDK_WROTL(z8, z8c, 8u);
+  uint32_t acc = z16 ? z16 : 0xffffffffu;
+  uint32_t len = z16 ? 16u : 0u;
+#define EDK_WSTEP(ZM, ZMC, M)                                    \
+  {                                                              \
+    uint32_t sh_ = len >= L ? len - L : len;                     \
+    sh_ = sh_ ? sh_ : L;                                         \
+    const uint32_t c_ = acc & EDK_WROTL(ZM, ZMC, sh_);           \
+    const bool t_ = c_ != 0;                                     \
+    acc = t_ ? c_ : acc;                                         \
+    len = t_ ? len + (M) : len;                                  \
+  }
+  EDK_WSTEP(z8, z8c, 8u)
+  EDK_WSTEP(z4, z4c, 4u)
+  EDK_WSTEP(z2, z2c, 2u)
+  EDK_WSTEP(z1, z1c, 1u)
+#undef EDK_WSTEP
+  const uint32_t top = acc & (0xffffffffu << c);  // one bit per ring position
+  if (__popc(top) > 2) return -1;
+  const uint32_t p1 = __builtin_ctz(top) - c;     // top != 0: acc is never 0
+  const uint32_t m2 = top & (top - 1);
+  const uint32_t p2 = m2 ? __builtin_ctz(m2) - c : p1;
+  // forward images: rotate right by ia = (p + 1) mod L
+  const uint32_t ia1 = (p1 + 1 == L) ? 0u : p1 + 1, ia2 = (p2 + 1 == L) ? 0u : p2 + 1;
+  const uint32_t va1 = EDK_WROTR(W, Wc, ia1), va2 = EDK_WROTR(W, Wc, ia2);
+  const uint32_t minA = min(va1, va2);
+  uint32_t period = L;
+  if ((L & 1) == 0 && EDK_WROTL(W, Wc, L >> 1) == W) period = L >> 1;
+  if (KIND == EDK_TRANSLATIONAL) {
+    rep = minA >> c;
+    // check_momentum: period * k' divisible by L, period in {L, L/2}
+    if (period != L && (B.kp & 1)) return 0;
+    const double R = B.Rtab[period - 1];
+    if (PH != 0) {
+      const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
+      const double2 cc = B.ph[Ma];
+      cr = cc.x * R;
+      ci = -cc.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    return 1;
+  }
+  // TranslationParityBasis: reflected images rev(rot_right^k(s)) =
+  // rotl^k(rev(s)), k = (p + 1 - len) mod L; rev(s) is brev32(s) in the top
+  // L bits, its window adds the continuation below
+  const uint32_t b = __brev(s);
+  const uint32_t Wb = b | (b >> L), Wbc = Wb << c;
+  int ib1 = (int)p1 + 1 - (int)len, ib2 = (int)p2 + 1 - (int)len;
+  ib1 += ib1 < 0 ? (int)L : 0;
+  ib1 += ib1 < 0 ? (int)L : 0;
+  ib2 += ib2 < 0 ? (int)L : 0;
+  ib2 += ib2 < 0 ? (int)L : 0;
+  const uint32_t vb1 = EDK_WROTL(Wb, Wbc, ib1 ? (uint32_t)ib1 : L);
+  const uint32_t vb2 = EDK_WROTL(Wb, Wbc, ib2 ? (uint32_t)ib2 : L);
+  const uint32_t minB = min(vb1, vb2);
+  const bool qp = (minA == minB);
+  const double R = B.Rtab[(qp ? L : 0) + period - 1];
+  double sgn = 1.0;
+  if (PH != 0) {
+    const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
+    const int Mb = (vb1 == minB && (vb2 != minB || ib1 < ib2)) ? ib1 : ib2;
+    if (minB < minA) sgn = (double)B.P * ((B.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (B.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
+  }
+  rep = (minA < minB ? minA : minB) >> c;
+  cr = sgn * R;
+  ci = 0.0;
+  return 1;
+#undef EDK_WROTL
+#undef EDK_WROTR
+}
+
 template <int KIND, int PH, int LT>
 __device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci) {
+  if constexpr (LT == 1) return canon_runs_w<KIND, PH>(B, s, rep, cr, ci);
   const int L = (LT == 32) ? 32 : B.L;
   const uint32_t mask = (LT == 32) ? 0xffffffffu : B.mask;
   const uint32_t z1 = ~s & mask;

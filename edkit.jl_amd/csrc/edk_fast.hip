@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && 
     s = A.keys[r0 + tid];
     const uint32_t ns = ~s & A.mask;
     for (int f = 0; f < T.nfam; f++) {
-      uint32_t r = LT == 32 ? __builtin_amdgcn_alignbit(s, s, 32 - T.fam[f].d) : rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      uint32_t r = rotr_lt<LT>(s, A.L - T.fam[f].d, A.L, A.mask);
       for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
     }
   }
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && 
     uint32_t o = off;
     for (int f = 0; f < T.nfam; f++) {
       const int d = T.fam[f].d;
-      uint32_t r = LT == 32 ? __builtin_amdgcn_alignbit(s, s, 32 - d) : rotr32(s, A.L - d, A.L, A.mask);
+      uint32_t r = rotr_lt<LT>(s, A.L - d, A.L, A.mask);
       for (int e = 0; e < T.fam[f].nent; e++) {
         const int flip = T.fam[f].ent[e].flip;
         uint32_t bits = ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask);
@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && 
     const uint32_t ns = ~s & A.mask;
     ST dg = 0;
     for (int f = 0; f < T.nfam; f++) {
-      uint32_t r = LT == 32 ? __builtin_amdgcn_alignbit(s, s, 32 - T.fam[f].d) : rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      uint32_t r = rotr_lt<LT>(s, A.L - T.fam[f].d, A.L, A.mask);
       uint32_t nr = ~r & A.mask;
       dg += (ST)(T.fam[f].diag[0] * __popc(ns & nr) + T.fam[f].diag[1] * __popc(ns & r) +
                  T.fam[f].diag[2] * __popc(s & nr) + T.fam[f].diag[3] * __popc(s & r));

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
     s = A.keys[r0 + tid];
     const uint32_t ns = ~s & A.mask;
     for (int f = 0; f < T.nfam; f++) {
-      uint32_t r = rotr32(s, A.L - T.fam[f].d, A.L, A.mask);
+      uint32_t r = rotr_lt<LT>(s, A.L - T.fam[f].d, A.L, A.mask);
       for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
     }
   }
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
     ST dg = 0;
     for (int f = 0; f < T.nfam; f++) {
       const int d = T.fam[f].d;
-      uint32_t r = rotr32(s, A.L - d, A.L, A.mask);
+      uint32_t r = rotr_lt<LT>(s, A.L - d, A.L, A.mask);
       uint32_t nr = ~r & A.mask;
       dg += (ST)(T.fam[f].diag[0] * __popc(ns & nr) + T.fam[f].diag[1] * __popc(ns & r) +
                  T.fam[f].diag[2] * __popc(s & nr) + T.fam[f].diag[3] * __popc(s & r));

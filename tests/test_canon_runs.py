@@ -107,3 +107,75 @@ def test_candidates_any_length_with_rotation_counts():
             assert (minB, Mb) == (min(refl), refl.index(min(refl))), (L, bin(s))
             per = next(p for p in range(1, L + 1) if L % p == 0 and rl(s, p) == s)
             assert per == (L // 2 if L % 2 == 0 and rl(s, L // 2) == s else L), (L, bin(s))
+
+
+def test_window_form_any_length():
+    """The window form used on the device for 16 <= L < 32: an L-bit ring word
+    x kept top-aligned in 32 bits with its low c = 32 - L bits continuing the
+    ring, so every rotation is one funnel shift of (W, W << c).  Same
+    representatives, rotation counts, reflected minima and periods as brute
+    force, for every L and filling."""
+    M32 = 0xffffffff
+    rnd = random.Random(5)
+
+    def alignbit(a, b, sh):
+        return (((a << 32) | b) >> (sh & 31)) & M32
+
+    def canon_w(s, L):
+        c = 32 - L
+        W = ((s << c) | (s >> (L - c))) & M32
+        Wc = (W << c) & M32
+
+        def wl(X, XC, m):  # rotl by 1 <= m <= L
+            return alignbit(X, XC, 32 - m)
+
+        def wr(X, XC, m):  # rotr by 0 <= m < L
+            return alignbit(X, XC, c + m)
+        z, zc = {1: ~W & M32}, {}
+        zc[1] = (z[1] << c) & M32
+        for m in (2, 4, 8, 16):
+            z[m] = z[m // 2] & wl(z[m // 2], zc[m // 2], m // 2)
+            zc[m] = (z[m] << c) & M32
+        acc, ln = (z[16], 16) if z[16] else (M32, 0)
+        for m in (8, 4, 2, 1):
+            sh = ln - L if ln >= L else ln
+            cc = acc & wl(z[m], zc[m], sh or L)
+            if cc:
+                acc, ln = cc, ln + m
+        top = acc & (M32 << c) & M32
+        cands = [b - c for b in range(c, 32) if top >> b & 1]
+        if len(cands) > 2:
+            return None
+        fa = [(wr(W, Wc, (p + 1) % L), (p + 1) % L) for p in cands]
+        b = brev(s)
+        Wb = b | (b >> L)
+        Wbc = (Wb << c) & M32
+        fb = [(wl(Wb, Wbc, (p + 1 - ln) % L or L), (p + 1 - ln) % L) for p in cands]
+        minA = min(v for v, _ in fa)
+        minB = min(v for v, _ in fb)
+        per = L // 2 if L % 2 == 0 and wl(W, Wc, L // 2) == W else L
+        return (minA >> c, min(k for v, k in fa if v == minA), minB >> c,
+                min(k for v, k in fb if v == minB), per)
+
+    for L in range(16, 32):
+        mask = (1 << L) - 1
+
+        def rl(x, m):
+            m %= L
+            return ((x << m) | (x >> (L - m))) & mask if m else x
+
+        def rv(x):
+            return int(f"{x:0{L}b}"[::-1], 2)
+        words = [rnd.getrandbits(L) for _ in range(150)]
+        words += [sum(1 << p for p in rnd.sample(range(L), n)) for n in range(1, L) for _ in range(6)]
+        if L % 2 == 0:
+            words += [sum(rnd.getrandbits(L // 2) << (k * (L // 2)) for k in range(2)) for _ in range(40)]
+        for s in words:
+            r = canon_w(s, L)
+            if r is None:
+                continue
+            imgs = [rl(s, (L - k) % L) for k in range(L)]
+            refl = [rv(v) for v in imgs]
+            per = next(p for p in range(1, L + 1) if L % p == 0 and rl(s, p) == s)
+            assert r[:4] == (min(imgs), imgs.index(min(imgs)), min(refl), refl.index(min(refl))), (L, bin(s))
+            assert r[4] == (L // 2 if L % 2 == 0 and (L // 2) % per == 0 else L), (L, bin(s))

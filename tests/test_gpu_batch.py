@@ -184,6 +184,10 @@ def test_l32_tile_schedule_matches_generic(gpu, kind, kw):
     ("TranslationalBasis", 30, dict(N=15, k=15), xxz),             # k = pi signs, L < 32
     ("TranslationParityBasis", 26, dict(N=13, k=13, p=-1), j1j2),  # signs, L < 32
     ("TranslationParityBasis", 24, dict(N=12, k=0, p=1), j1j2),    # positive, L < 32
+    ("TranslationalBasis", 21, dict(N=10, k=4), xxz),              # odd L, complex phases
+    ("TranslationParityBasis", 19, dict(N=9, k=0, p=-1), j1j2),    # odd L, parity
+    ("TranslationalBasis", 16, dict(N=5, k=2), j1j2),              # L = 16, low filling
+    ("TranslationalBasis", 31, dict(N=11, k=7), xxz),              # L = 31
 ])
 def test_run_length_paths_match_generic(gpu, kind, L, kw, model):
     """The run-length canonicaliser (tile kernel for one vector, block kernel
