@@ -435,6 +435,58 @@ __device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint3
 #undef EDK_WROTR
 }
 
+// Full search on periodic windows (16 <= L < 32, unit cell 1) for the words
+// canon_runs_w defers: every image is one v_alignbit; the first strict
+// minimum over increasing rotation counts gives the reference's phase index,
+// and the first count that maps the word onto itself is its period.
+template <int KIND, int PH>
+__device__ __forceinline__ bool fast_canon_w(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr,
+                                             double &ci) {
+  const uint32_t L = (uint32_t)B.L, c = 32u - L;
+  const uint32_t W = (s << c) | (s >> (L - c)), Wc = W << c;
+  uint32_t minA = W, Ma = 0, period = L;
+  for (uint32_t m = 1; m < L; m++) {
+    const uint32_t v = __builtin_amdgcn_alignbit(W, Wc, c + m);  // rotr_L(s, m)
+    Ma = v < minA ? m : Ma;
+    minA = min(minA, v);
+    period = (v == W && period == L) ? m : period;
+  }
+  if (KIND == EDK_TRANSLATIONAL) {
+    rep = minA >> c;
+    if (((int64_t)period * B.kp) % (int64_t)L) return false;  // check_momentum
+    const double R = B.Rtab[period - 1];
+    if (PH != 0) {
+      const double2 cc = B.ph[Ma];
+      cr = cc.x * R;
+      ci = -cc.y * R;
+    } else {
+      cr = R;
+      ci = 0.0;
+    }
+    return true;
+  }
+  // reflected images rev(rotr^k(s)) = rotl^k(rev(s)), k = 0 .. L-1
+  const uint32_t b = __brev(s);
+  const uint32_t Wb = b | (b >> L), Wbc = Wb << c;
+  uint32_t minB = Wb, Mb = 0;
+  for (uint32_t k = 1; k < L; k++) {
+    const uint32_t v = __builtin_amdgcn_alignbit(Wb, Wbc, 32u - k);  // rotl_L(rev(s), k)
+    Mb = v < minB ? k : Mb;
+    minB = min(minB, v);
+  }
+  const bool qp = (minA == minB);
+  const double R = B.Rtab[(qp ? L : 0) + period - 1];
+  double sgn = 1.0;
+  if (PH != 0) {
+    if (minB < minA) sgn = (double)B.P * ((B.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
+    else sgn = (B.kp == 1 || (Ma & 1) == 0) ? 1.0 : -1.0;
+  }
+  rep = (minA < minB ? minA : minB) >> c;
+  cr = sgn * R;
+  ci = 0.0;
+  return true;
+}
+
 template <int KIND, int PH, int LT>
 __device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci) {
   if constexpr (LT == 1) return canon_runs_w<KIND, PH>(B, s, rep, cr, ci);

@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && 
       uint32_t r = 0;
       double cr = 0.0, ci = 0.0;
       const bool okb = (LT == 32) ? fast_canon32<KIND, PH>(B, tgt[j], r, cr, ci)
-                                  : fast_canon<KIND, PH>(B, tgt[j], r, cr, ci);
+                                  : fast_canon_w<KIND, PH>(B, tgt[j], r, cr, ci);
       if (okb) {
         tgt[j] = r;
         const double v = sV[ent[j]];

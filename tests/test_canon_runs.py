@@ -179,3 +179,53 @@ def test_window_form_any_length():
             per = next(p for p in range(1, L + 1) if L % p == 0 and rl(s, p) == s)
             assert r[:4] == (min(imgs), imgs.index(min(imgs)), min(refl), refl.index(min(refl))), (L, bin(s))
             assert r[4] == (L // 2 if L % 2 == 0 and (L // 2) % per == 0 else L), (L, bin(s))
+
+
+def test_window_full_search_any_length():
+    """The deferred words' full search on periodic windows (16 <= L < 32): the
+    first strict minimum over increasing rotation counts is the reference's
+    phase index, the first count mapping the word onto itself its period;
+    checked against brute force on random and periodic words."""
+    M32 = 0xffffffff
+    rnd = random.Random(9)
+
+    def alignbit(a, b, sh):
+        return (((a << 32) | b) >> (sh & 31)) & M32
+
+    for L in range(16, 32):
+        mask, c = (1 << L) - 1, 32 - L
+
+        def rl(x, m):
+            m %= L
+            return ((x << m) | (x >> (L - m))) & mask if m else x
+
+        def rv(x):
+            return int(f"{x:0{L}b}"[::-1], 2)
+        words = [rnd.getrandbits(L) for _ in range(100)]
+        for p in range(1, L):
+            if L % p == 0:
+                for _ in range(10):
+                    u = rnd.getrandbits(p)
+                    words.append(sum(u << (k * p) for k in range(L // p)))
+        for s in words:
+            W = ((s << c) | (s >> (L - c))) & M32
+            Wc = (W << c) & M32
+            minA, Ma, per = W, 0, L
+            for m in range(1, L):
+                v = alignbit(W, Wc, c + m)
+                Ma = m if v < minA else Ma
+                minA = min(minA, v)
+                per = m if (v == W and per == L) else per
+            b = brev(s)
+            Wb = b | (b >> L)
+            Wbc = (Wb << c) & M32
+            minB, Mb = Wb, 0
+            for k in range(1, L):
+                v = alignbit(Wb, Wbc, 32 - k)
+                Mb = k if v < minB else Mb
+                minB = min(minB, v)
+            imgs = [rl(s, (L - k) % L) for k in range(L)]
+            refl = [rv(x) for x in imgs]
+            assert (minA >> c, Ma) == (min(imgs), imgs.index(min(imgs))), (L, bin(s))
+            assert (minB >> c, Mb) == (min(refl), refl.index(min(refl))), (L, bin(s))
+            assert per == next(p for p in range(1, L + 1) if L % p == 0 and rl(s, p) == s), (L, bin(s))
