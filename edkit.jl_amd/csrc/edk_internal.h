@@ -38,19 +38,23 @@ struct edk_ctx {
   hipStream_t stream;
   bool own_stream;
   int num_cus;
-  // stream-ordered allocation (hipMallocAsync/hipFreeAsync on `stream`) for
-  // the Krylov drivers' per-call buffers: a plain hipFree waits for the whole
-  // device, which would serialise contexts that run side by side
-  bool async_alloc = false;
+  // cached allocation for the Krylov drivers' per-call buffers: blocks come
+  // from a per-device cache of size classes and go back to it after this
+  // context's stream has finished with them.  A plain hipFree waits for the
+  // whole device (serialising contexts that run side by side), and fresh
+  // device memory costs ~20-70 GB/s to map.
+  bool cache_alloc = false;
   // two pinned slots of EDK_PINNED_SLOT double2 for reduction results, owned
   // by the context (no pinned allocation per call)
   void *pinned = nullptr;
 };
 #define EDK_PINNED_SLOT 1040
 
-// device buffers of a context's calls (see edk_ctx::async_alloc)
+// device buffers of a context's calls (see edk_ctx::cache_alloc)
 hipError_t edk_dalloc(edk_ctx *c, void **p, size_t bytes);
 void edk_dfree(edk_ctx *c, void *p);
+// bytes the block cache of a device holds for reuse
+size_t edk_cache_bytes(int device);
 
 // Generator kinds of the Abelian canonicaliser.
 enum { GEN_ROTL = 0, GEN_REV = 1, GEN_IDENT = 2, GEN_BENES = 3 };

@@ -501,8 +501,11 @@ int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, 
     return rc;
   }
   size_t es = K.esize();
+  // the basis grows by doubling from 64 columns (most solves stop well
+  // before m): smaller, reusable allocations instead of m + 1 columns at once
+  int cap = std::min(m + 1, 64);
   void *V = nullptr, *w = nullptr;
-  if (edk_dalloc(ctx, &V, (size_t)D * (m + 1) * es) != hipSuccess || edk_dalloc(ctx, &w, (size_t)D * es) != hipSuccess) {
+  if (edk_dalloc(ctx, &V, (size_t)D * cap * es) != hipSuccess || edk_dalloc(ctx, &w, (size_t)D * es) != hipSuccess) {
     edk_dfree(ctx, V);
     K.release();
     return edk_fail(EDK_ERR_OOM, "Lanczos basis does not fit in device memory");
@@ -520,6 +523,23 @@ int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, 
   PartialOrth po(dtype == EDK_C64 ? 2.220446049250313e-16 : 1.1920928955078125e-07);
   PartialOrth *pop = reorth == EDK_REORTH_PARTIAL ? &po : nullptr;
   for (int j = 1; !rc && j <= m; j++) {
+    if (j + 1 > cap) {
+      // step j writes column j: move columns 0..j-1 to a basis twice as wide
+      const int ncap = std::min(m + 1, 2 * cap);
+      void *V2 = nullptr;
+      if (edk_dalloc(ctx, &V2, (size_t)D * ncap * es) != hipSuccess) {
+        rc = edk_fail(EDK_ERR_OOM, "Lanczos basis does not fit in device memory");
+        break;
+      }
+      if (hipMemcpyAsync(V2, V, (size_t)D * j * es, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
+        edk_dfree(ctx, V2);
+        rc = edk_fail(EDK_ERR_HIP, "Lanczos basis copy");
+        break;
+      }
+      edk_dfree(ctx, V);
+      V = V2;
+      cap = ncap;
+    }
     int br = lanczos_extend(op, K, V, D, w, alpha.data(), beta.data(), &mm, j, &mv, m + 1, pop);
     if (br < 0) {
       rc = -br;
@@ -575,7 +595,6 @@ int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, 
   edk_dfree(ctx, V);
   edk_dfree(ctx, w);
   K.release();
-  if (ctx->async_alloc) hipStreamSynchronize(ctx->stream);
   if (rc) return rc;
   *E0 = e0;
   *resid = res;
@@ -797,7 +816,6 @@ static void krylov_free(edk_krylov *c) {
   edk_dfree(c->ctx, c->V);
   edk_dfree(c->ctx, c->w);
   c->K.release();
-  if (c->ctx->async_alloc) (void)hipStreamSynchronize(c->ctx->stream);
   delete c;
 }
 

@@ -1,5 +1,8 @@
 // edk_runtime.hip -- contexts, errors and device-memory helpers of the C ABI.
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 
 #include "edk_internal.h"
 
@@ -47,16 +50,97 @@ extern "C" int edk_ctx_create(int device, void *stream, edk_ctx **out) {
   return EDK_OK;
 }
 
+// Cached allocations are rounded up to size classes (1/8 of the enclosing
+// power of two above 64 MiB, at most 12.5 % extra), so a block freed by one
+// job serves the next job of a similar sector.
+static size_t size_class(size_t b) {
+  if (b < ((size_t)64 << 20)) return (b + 255) & ~(size_t)255;
+  const int e = 63 - __builtin_clzll((unsigned long long)b);
+  const size_t g = (size_t)1 << (e - 3);
+  return (b + g - 1) & ~(g - 1);
+}
+
+// Per-device cache of free blocks by size class.  Blocks stay cached for the
+// life of the process; an allocation that finds the device full releases the
+// cache and retries.
+struct BlockCache {
+  std::mutex mu;
+  std::map<size_t, std::vector<void *>> free_blocks;
+  std::unordered_map<void *, size_t> owned;  // every block the cache handed out or holds
+  size_t cached = 0;
+};
+static std::mutex g_caches_mu;
+static std::map<int, BlockCache *> g_caches;
+
+static BlockCache &cache_of(int device) {
+  std::lock_guard<std::mutex> lk(g_caches_mu);
+  BlockCache *&bc = g_caches[device];
+  if (!bc) bc = new BlockCache();
+  return *bc;
+}
+
+static void cache_release(BlockCache &bc) {
+  std::lock_guard<std::mutex> lk(bc.mu);
+  for (auto &kv : bc.free_blocks)
+    for (void *q : kv.second) {
+      (void)hipFree(q);
+      bc.owned.erase(q);
+    }
+  bc.free_blocks.clear();
+  bc.cached = 0;
+}
+
+size_t edk_cache_bytes(int device) {
+  BlockCache &bc = cache_of(device);
+  std::lock_guard<std::mutex> lk(bc.mu);
+  return bc.cached;
+}
+
 hipError_t edk_dalloc(edk_ctx *c, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
-  if (c->async_alloc) return hipMallocAsync(p, bytes, c->stream);
-  return hipMalloc(p, bytes);
+  if (!c->cache_alloc) return hipMalloc(p, bytes);
+  const size_t b = size_class(bytes);
+  BlockCache &bc = cache_of(c->device);
+  {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    auto it = bc.free_blocks.find(b);
+    if (it != bc.free_blocks.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      bc.cached -= b;
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipMalloc(p, b);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    cache_release(bc);
+    e = hipMalloc(p, b);
+  }
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(bc.mu);
+    bc.owned[*p] = b;
+  }
+  return e;
 }
 
 void edk_dfree(edk_ctx *c, void *p) {
   if (!p) return;
-  if (c->async_alloc) (void)hipFreeAsync(p, c->stream);
-  else (void)hipFree(p);
+  if (!c->cache_alloc) {
+    (void)hipFree(p);
+    return;
+  }
+  // the block's last use was queued on this context's stream
+  (void)hipStreamSynchronize(c->stream);
+  BlockCache &bc = cache_of(c->device);
+  std::lock_guard<std::mutex> lk(bc.mu);
+  auto it = bc.owned.find(p);
+  if (it == bc.owned.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  bc.free_blocks[it->second].push_back(p);
+  bc.cached += it->second;
 }
 
 extern "C" int edk_ctx_destroy(edk_ctx *c) {

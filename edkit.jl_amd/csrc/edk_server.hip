@@ -1,9 +1,9 @@
 // edk_server.hip -- serving loop: Lanczos ground-state and Krylov time-
 // evolution jobs on one device, admitted as worker slots and HBM free up.
 //
-// Each worker thread owns a context (its own HIP stream, stream-ordered
-// allocation), so the jobs it runs overlap on the device with the other
-// workers' jobs.  The queue is first-in first-out: its head is admitted when
+// Each worker thread owns a context (its own HIP stream, buffers from the
+// device's block cache), so the jobs it runs overlap on the device with the
+// other workers' jobs.  The queue is first-in first-out: its head is admitted when
 // a worker is idle and the head's device footprint fits beside the jobs
 // already running (a job that is larger than the whole budget runs when
 // nothing else does).  A job's arithmetic depends only on its own inputs --
@@ -59,7 +59,17 @@ static size_t job_footprint(const edk_job &j) {
   const size_t D = (size_t)j.op->basis->D, es = job_esize(j);
   const size_t parts = (size_t)1024 * (EDK_KRYLOV_MAXM + 2) * 16;
   if (j.kind == EDK_JOB_GROUND) {
-    size_t f = D * es * ((size_t)j.m + 3) + parts;
+    // the Lanczos basis grows 64 -> 128 -> ... -> m + 1 columns; while it
+    // grows, the old and the new basis coexist
+    size_t cols = 0;
+    for (size_t cap = std::min<size_t>(j.m + 1, 64); cap < (size_t)j.m + 1;) {
+      const size_t ncap = std::min<size_t>(j.m + 1, 2 * cap);
+      cols = std::max(cols, cap + ncap);
+      cap = ncap;
+    }
+    cols = std::max<size_t>(cols, std::min<size_t>(j.m + 1, 64));
+    // cached allocations round up by at most 1/8
+    size_t f = D * es * (cols + 2) / 8 * 9 + parts;
     if (j.refine || j.want_vector) f += 2 * D * 16;
     if (j.want_vector) f += D * es;
     return f;
@@ -189,16 +199,10 @@ extern "C" int edk_server_create(int device, int workers, size_t hbm_budget, edk
   if (device < 0 || device >= n) return edk_fail(EDK_ERR_ARG, "edk_server_create: no such device");
   EDK_HIP(hipSetDevice(device));
   if (hbm_budget == 0) {
+    // 80 % of what is free, counting the blocks cached for reuse as free
     size_t fr = 0, tot = 0;
     EDK_HIP(hipMemGetInfo(&fr, &tot));
-    hbm_budget = fr / 10 * 8;
-  }
-  // keep freed job memory in the device pool instead of returning it to the
-  // driver at every synchronisation (the admission budget bounds the pool)
-  hipMemPool_t pool;
-  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-    uint64_t thr = UINT64_MAX;
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    hbm_budget = (fr + edk_cache_bytes(device)) / 10 * 8;
   }
   edk_server *S = new edk_server();
   S->device = device;
@@ -212,7 +216,7 @@ extern "C" int edk_server_create(int device, int workers, size_t hbm_budget, edk
       delete S;
       return rc;
     }
-    c->async_alloc = true;
+    c->cache_alloc = true;
     S->ctxs.push_back(c);
   }
   for (int w = 0; w < workers; w++) S->threads.emplace_back(worker_main, S, w);
