@@ -75,6 +75,70 @@ __device__ int edk_judge(const CanonDev &C, uint64_t s, double *nrm) {
     *nrm = C.Rtab[(qp ? C.len : 0) + R - 1];
     return 1;
   }
+  if (C.kind == EDK_PARITY || C.kind == EDK_FLIP) {  // ParityBasis.jl:44-57, FlipBasis.jl:42-55
+    const uint64_t In = C.kind == EDK_PARITY ? edk_revfields(s, C.L, C.w) : edk_flipfields(C, s);
+    if (In < s) return 0;
+    if (In == s) {
+      if (C.P != 1) return 0;
+      *nrm = C.Rtab[1];
+    } else {
+      *nrm = C.Rtab[0];
+    }
+    return 1;
+  }
+  if (C.kind == EDK_PARITY_FLIP) {  // double_parity_check, ParityFlipBasis.jl:47-65, 73-79
+    const uint64_t Ip = edk_revfields(s, C.L, C.w);
+    if (Ip < s) return 0;
+    const uint64_t Iz = edk_flipfields(C, s);
+    if (Iz < s) return 0;
+    const uint64_t Ipz = edk_flipfields(C, Ip);
+    if (Ipz < s) return 0;
+    int N = 1;
+    if (Ip == s) {
+      if (C.P != 1) return 0;
+      N++;
+    }
+    if (Iz == s) {
+      if (C.Z != 1) return 0;
+      N++;
+    }
+    if (Ipz == s) {
+      if (C.P * C.Z != 1) return 0;
+      N++;
+    }
+    *nrm = C.Rtab[N - 1];
+    return 1;
+  }
+  if (C.kind == EDK_TRANSLATION_FLIP) {  // TranslationFlipJudge, TranslationalFlipBasis.jl:50-94
+    int R = C.len;
+    bool qp = false;
+    const uint64_t In = edk_flipfields(C, s);
+    if (In < s) return 0;
+    if (In == s) {
+      qp = true;
+      if (C.P != 1) return 0;
+    }
+    const int half = C.len / 2;
+    uint64_t st = s;
+    for (int n = 1; n < C.len; n++) {
+      st = edk_rotr(st, sh, tot, C.mask);  // circshift!(dgt, A)
+      const uint64_t In2 = edk_flipfields(C, st);
+      if (st < s) return 0;
+      if (In2 < s) return 0;
+      if (In2 == s) {
+        // isequal(isone(P), iseven(div(n * K, L / 2)))
+        if ((C.P == 1) != (((n * C.kp) / half) % 2 == 0)) return 0;
+        qp = true;
+      }
+      if (st == s) {
+        if ((n * C.kp) % C.len) return 0;
+        R = n;
+        break;
+      }
+    }
+    *nrm = C.Rtab[(qp ? C.len : 0) + R - 1];
+    return 1;
+  }
   // check_min_int / check_min, AbelianBasis.jl:322-338, 399-418
   int os[EDK_MAXGEN];
   for (int g = 0; g < C.ngen; g++) os[g] = 1;
@@ -318,6 +382,7 @@ CanonDev edk_basis::canon() const {
   c.lowfields = lf;
   c.kp = kp;
   c.P = P;
+  c.Z = Z;
   c.ngen = ngen;
   c.order = order;
   c.Rtab = d_Rtab;
@@ -565,6 +630,43 @@ static int setup_basis(edk_ctx *ctx, const edk_basis_spec *spec, edk_basis *b) {
       b->h_Rtab[n - 1] = n2 / std::sqrt(2.0);
     }
     b->order = 2 * b->L;
+  } else if (kind == EDK_PARITY || kind == EDK_FLIP || kind == EDK_PARITY_FLIP || kind == EDK_TRANSLATION_FLIP) {
+    if (spec->p != 1 && spec->p != -1) return edk_fail(EDK_ERR_ARG, "Invalid parity");
+    b->P = spec->p;
+    if (kind == EDK_PARITY_FLIP) {
+      if (spec->z != 1 && spec->z != -1) return edk_fail(EDK_ERR_ARG, "Invalid parity");
+      b->Z = spec->z;
+    }
+    // the flip maps charge N to L(base-1) - N (FlipBasis.jl:77, ParityFlipBasis.jl:92,
+    // TranslationalFlipBasis.jl:129)
+    if (kind != EDK_PARITY && spec->N >= 0 && 2 * spec->N != b->L * (b->base - 1))
+      return edk_fail(EDK_ERR_ARG, "N not compatible with spin flip.");
+    if (kind == EDK_PARITY || kind == EDK_FLIP) {
+      b->h_Rtab = {std::sqrt(2.0), 2.0};  // ParityJudge / FlipJudge C = sqrt(2); 2.0 for a self-image
+      b->order = 2;
+    } else if (kind == EDK_PARITY_FLIP) {
+      b->h_Rtab = {2.0, 2.0 * std::sqrt(2.0), 0.0, 4.0};  // ParityFlipBasis.jl:96
+      b->order = 4;
+    } else {
+      const int len = b->len;
+      b->kp = ((-spec->k) % len + len) % len;  // TranslationalFlipBasis.jl:136
+      b->h_Rtab.resize(2 * len);
+      for (int n = 1; n <= len; n++) {  // N2 = 2len/sqrt(i), N1 = N2/sqrt(2)  (:140-141)
+        const double n2 = (double)(2 * len) / std::sqrt((double)n);
+        b->h_Rtab[len + n - 1] = n2;
+        b->h_Rtab[n - 1] = n2 / std::sqrt(2.0);
+      }
+      b->h_ph.resize(len);
+      for (int m = 0; m < len; m++) {  // phase_factor(k, len), TranslationalBasis.jl:277-285
+        if (b->kp == 0) b->h_ph[m] = make_double2(1.0, 0.0);
+        else if (2 * b->kp == len) b->h_ph[m] = make_double2((m % 2 == 0) ? 1.0 : -1.0, 0.0);
+        else {
+          const double th = ((-2.0 * M_PI / len) * b->kp) * m;
+          b->h_ph[m] = make_double2(cos(th), sin(th));
+        }
+      }
+      b->order = 2 * b->L;  // TranslationalFlipBasis.jl:222
+    }
   } else if (kind == EDK_PROJECTED) {
     if (spec->N < 0) return edk_fail(EDK_ERR_UNSUPPORTED, "ProjectedBasis needs N (a predicate f is applied on the host)");
   } else if (kind != EDK_TENSOR && kind != EDK_ABELIAN) {

@@ -2,15 +2,17 @@
 path, driven from Python through the C ABI of libedkit_hip.so.
 
 The names mirror the Julia API: bases (TensorBasis, ProjectedBasis,
-TranslationalBasis, TranslationParityBasis, basis), spin, operator,
+TranslationalBasis, TranslationParityBasis, ParityBasis, FlipBasis,
+ParityFlipBasis, TranslationFlipBasis, basis), spin, operator,
 trans_inv_operator, mul_ (mul!), H @ x / H * x, mul, timeevolve,
 KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, plus the new
 lanczos_ground.
 """
 from ._lib import EdkError, lib  # noqa: F401
 from .batch import MatvecBatch  # noqa: F401
-from .basis import (AbelianBasis, AbstractBasis, ProjectedBasis, TensorBasis,  # noqa: F401
-                    TranslationalBasis, TranslationParityBasis, basis, productstate)
+from .basis import (AbelianBasis, AbstractBasis, FlipBasis, ParityBasis,  # noqa: F401
+                    ParityFlipBasis, ProjectedBasis, TensorBasis, TranslationalBasis,
+                    TranslationFlipBasis, TranslationParityBasis, basis, productstate)
 from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
 from .matvec import (Array, apply, clear_sparse_cache_, device_operator, matvec_into, mul,  # noqa: F401

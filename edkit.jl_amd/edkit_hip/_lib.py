@@ -11,7 +11,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libedkit_hip.so")
 
 EDK_OK = 0
-EDK_TENSOR, EDK_PROJECTED, EDK_TRANSLATIONAL, EDK_TRANSLATION_PARITY, EDK_ABELIAN = range(5)
+(EDK_TENSOR, EDK_PROJECTED, EDK_TRANSLATIONAL, EDK_TRANSLATION_PARITY, EDK_ABELIAN, EDK_PARITY, EDK_FLIP,
+ EDK_PARITY_FLIP, EDK_TRANSLATION_FLIP) = range(9)
 EDK_C64, EDK_C32 = 0, 1
 (EDK_VARIANT_AUTO, EDK_VARIANT_GENERIC, EDK_VARIANT_TILE, EDK_VARIANT_LANE2, EDK_VARIANT_LANE4,
  EDK_VARIANT_LANE8) = range(6)

@@ -4,6 +4,10 @@ TensorBasis           AbstractBasis.jl:217-247
 ProjectedBasis        ProjectedBasis.jl:244-260
 TranslationalBasis    TranslationalBasis.jl:235-266
 TranslationParityBasis TranslationalParityBasis.jl:141-175
+ParityBasis           ParityBasis.jl:76-97
+FlipBasis             FlipBasis.jl:72-95
+ParityFlipBasis       ParityFlipBasis.jl:87-111
+TranslationFlipBasis  TranslationalFlipBasis.jl:122-158
 basis(...)            AbelianBasis.jl:771-847
 A predicate `f` is evaluated on the host over the device-built table (the
 reference evaluates it on the unshifted candidate, i.e. the representative,
@@ -147,6 +151,43 @@ class TranslationalBasis(AbstractBasis):
 class TranslationParityBasis(AbstractBasis):
     def __init__(self, L, k=0, p=1, N=None, a=1, f=None, base=2, ctx=None):
         super().__init__(_lib.EDK_TRANSLATION_PARITY, L, base, a=a, N=N, k=k, p=p, f=f, ctx=ctx)
+
+
+class ParityBasis(AbstractBasis):
+    """ParityBasis(; L, f, p, N, base): one representative per reflection orbit."""
+    complex_phases = True  # eltype(::AbstractBasis) = ComplexF64 (AbstractBasis.jl:121)
+
+    def __init__(self, L, p, N=None, f=None, base=2, ctx=None):
+        super().__init__(_lib.EDK_PARITY, L, base, N=N, p=p, f=f, ctx=ctx)
+
+
+class FlipBasis(AbstractBasis):
+    """FlipBasis(; L, f, p, N, base): one representative per spin-flip orbit
+    (digit d -> base - 1 - d); N must be L(base - 1)/2 when given."""
+    complex_phases = True
+
+    def __init__(self, L, p, N=None, f=None, base=2, ctx=None):
+        super().__init__(_lib.EDK_FLIP, L, base, N=N, p=p, f=f, ctx=ctx)
+
+
+class ParityFlipBasis(AbstractBasis):
+    """ParityFlipBasis(; L, f, p, z, N, base): reflection and spin flip."""
+    complex_phases = True
+
+    def __init__(self, L, p, z, N=None, f=None, base=2, ctx=None):
+        super().__init__(_lib.EDK_PARITY_FLIP, L, base, N=N, p=p, z=z, f=f, ctx=ctx)
+
+
+class TranslationFlipBasis(AbstractBasis):
+    """TranslationFlipBasis(; f, k, p, L, N, a, base): translation by the unit
+    cell and spin flip with eigenvalue p."""
+
+    def __init__(self, L, k=0, p=1, N=None, a=1, f=None, base=2, ctx=None):
+        super().__init__(_lib.EDK_TRANSLATION_FLIP, L, base, a=a, N=N, k=k, p=p, f=f, ctx=ctx)
+        ln = self.L // self.a
+        kp = (-k) % ln
+        # eltype follows phase_factor: real at k' in {0, len/2}
+        self.complex_phases = not (kp == 0 or 2 * kp == ln)
 
 
 class AbelianBasis(AbstractBasis):

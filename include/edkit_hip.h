@@ -46,6 +46,10 @@ extern "C" {
 #define EDK_TRANSLATIONAL 2
 #define EDK_TRANSLATION_PARITY 3
 #define EDK_ABELIAN 4
+#define EDK_PARITY 5
+#define EDK_FLIP 6
+#define EDK_PARITY_FLIP 7
+#define EDK_TRANSLATION_FLIP 8
 
 /* vector element types */
 #define EDK_C64 0 /* ComplexF64 */
@@ -84,6 +88,12 @@ int edk_memset(edk_ctx *ctx, void *dptr, int value, size_t bytes);
  *                                                           TranslationalParityBasis.jl:141-175
  *   EDK_ABELIAN             basis(L, N, k, a, p, z, symmetries, base)
  *                                                           AbelianBasis.jl:771-847
+ *   EDK_PARITY              ParityBasis(L, p, N, base)      ParityBasis.jl:76-97
+ *   EDK_FLIP                FlipBasis(L, p, N, base)        FlipBasis.jl:72-95
+ *   EDK_PARITY_FLIP         ParityFlipBasis(L, p, z, N, base)
+ *                                                           ParityFlipBasis.jl:87-111
+ *   EDK_TRANSLATION_FLIP    TranslationFlipBasis(L, k, p, N, a, base)
+ *                                                           TranslationalFlipBasis.jl:122-158
  * For EDK_ABELIAN has_k/has_p/has_z tell which keywords were given; custom
  * generators come as ncustom 1-based permutations of length L (perms),
  * momenta (ks) and optional spin-inversion flags (invs, ncustom*L bytes). */

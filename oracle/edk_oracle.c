@@ -136,7 +136,8 @@ int orc_compute_period(const int32_t *perm, const uint8_t *inv, int L) {
 struct orc_basis {
   int kind, L, base, A, len;
   int kp;  /* Translational: k' = mod(-k, len); TP: K = +-1 */
-  int P;   /* TP parity */
+  int P;   /* TP parity; reflection or flip eigenvalue of the flip bases */
+  int Z;   /* ParityFlipBasis: spin-flip eigenvalue */
   int Nnum; /* digit-sum constraint (-1 none) */
   int64_t D;
   int64_t *tab; /* 1-based contents (b.I) */
@@ -160,6 +161,10 @@ int orc_basis_order(const orc_basis *b) {
     case ORC_TRANSLATIONAL: return b->L;        /* TranslationalBasis.jl:349 */
     case ORC_TRANSLATION_PARITY: return 2 * b->L; /* TranslationalParityBasis.jl:241 */
     case ORC_ABELIAN: return b->order;           /* AbelianBasis.jl:507 */
+    case ORC_PARITY: return 2;                   /* ParityBasis.jl:128 */
+    case ORC_FLIP: return 2;                     /* FlipBasis.jl:122 */
+    case ORC_PARITY_FLIP: return 4;              /* ParityFlipBasis.jl:152 */
+    case ORC_TRANSLATION_FLIP: return 2 * b->L;  /* TranslationalFlipBasis.jl:222 */
     default: return 1;                           /* AbstractBasis.jl:197 */
   }
 }
@@ -197,6 +202,9 @@ static inline uint64_t reverse_digits(const orc_basis *b, uint64_t s) {
   }
   return r;
 }
+
+/* _int_spinflip (AbstractBasis.jl:434) == MAX - i with MAX = base^L + 1 */
+static inline uint64_t spinflip(const orc_basis *b, uint64_t s) { return b->pw[b->L] - 1 - s; }
 
 static inline int digit_sum(const orc_basis *b, uint64_t s) {
   int t = 0;
@@ -334,12 +342,87 @@ static int judge_abelian(const orc_basis *b, uint64_t s, double *nrm) {
   return 1;
 }
 
+/* ParityJudge (ParityBasis.jl:44-57) and FlipJudge (FlipBasis.jl:42-55) */
+static int judge_pair(const orc_basis *b, uint64_t s, double *nrm) {
+  uint64_t In = (b->kind == ORC_PARITY) ? reverse_digits(b, s) : spinflip(b, s);
+  if (In < s) return 0;
+  if (In == s) {
+    if (b->P != 1) return 0;
+    *nrm = 2.0;
+  } else {
+    *nrm = sqrt(2.0);
+  }
+  return 1;
+}
+
+/* double_parity_check + ParityFlipJudge (ParityFlipBasis.jl:47-65, 73-79) */
+static int judge_parityflip(const orc_basis *b, uint64_t s, double *nrm) {
+  static const double Cn[4] = {2.0, 2.0 * 1.4142135623730951, 0.0, 4.0}; /* :96 */
+  uint64_t Ip = reverse_digits(b, s);
+  if (Ip < s) return 0;
+  uint64_t Iz = spinflip(b, s);
+  if (Iz < s) return 0;
+  uint64_t Ipz = spinflip(b, Ip);
+  if (Ipz < s) return 0;
+  int N = 1;
+  if (Ip == s) {
+    if (b->P != 1) return 0;
+    N++;
+  }
+  if (Iz == s) {
+    if (b->Z != 1) return 0;
+    N++;
+  }
+  if (Ipz == s) {
+    if (b->P * b->Z != 1) return 0;
+    N++;
+  }
+  *nrm = Cn[N - 1];
+  return 1;
+}
+
+/* TranslationFlipJudge (TranslationalFlipBasis.jl:50-94) */
+static int judge_transflip(const orc_basis *b, uint64_t s, double *nrm) {
+  int R = b->len, Qp = 0;
+  uint64_t In = spinflip(b, s);
+  if (In < s) return 0;
+  if (In == s) {
+    Qp = 1;
+    if (b->P != 1) return 0;
+  }
+  uint64_t st = s;
+  for (int n = 1; n <= b->len - 1; n++) {
+    st = translate(b, st); /* circshift!(dgt, A) */
+    uint64_t In1 = st, In2 = spinflip(b, st);
+    if (In1 < s) return 0;
+    if (In2 < s) return 0;
+    if (In2 == s) {
+      /* isequal(isone(P), iseven(div(n * K, L / 2))) */
+      int ev = ((n * b->kp) / (b->len / 2)) % 2 == 0;
+      if ((b->P == 1) != ev) return 0;
+      Qp = 1;
+    }
+    if (In1 == s) {
+      if ((n * b->kp) % b->len != 0) return 0;
+      R = n;
+      break;
+    }
+  }
+  double n2 = (double)(2 * b->len) / sqrt((double)R); /* N2, N1 = N2/sqrt(2) (:140-141) */
+  *nrm = Qp ? n2 : n2 / sqrt(2.0);
+  return 1;
+}
+
 static int judge(const orc_basis *b, uint64_t s, double *nrm) {
   switch (b->kind) {
     case ORC_PROJECTED: *nrm = 1.0; return 1;
     case ORC_TRANSLATIONAL: return judge_trans(b, s, nrm);
     case ORC_TRANSLATION_PARITY: return judge_tp(b, s, nrm);
     case ORC_ABELIAN: return judge_abelian(b, s, nrm);
+    case ORC_PARITY:
+    case ORC_FLIP: return judge_pair(b, s, nrm);
+    case ORC_PARITY_FLIP: return judge_parityflip(b, s, nrm);
+    case ORC_TRANSLATION_FLIP: return judge_transflip(b, s, nrm);
   }
   return 0;
 }
@@ -575,6 +658,32 @@ orc_basis *orc_basis_build(int kind, int L, int base, int a, int N, int k,
     b->P = p;
     b->C = (cplx *)malloc(b->len * sizeof(cplx));
     for (int i = 0; i < b->len; i++) b->C[i] = (b->kp == 1 || i % 2 == 0) ? 1.0 : -1.0;
+  } else if (kind == ORC_PARITY || kind == ORC_FLIP || kind == ORC_PARITY_FLIP ||
+             kind == ORC_TRANSLATION_FLIP) {
+    if ((p != 1 && p != -1) || (kind == ORC_PARITY_FLIP && z != 1 && z != -1)) {
+      set_err("Invalid parity");
+      free(b);
+      return NULL;
+    }
+    if (kind != ORC_PARITY && N >= 0 && 2 * N != L * (base - 1)) {
+      set_err("N not compatible."); /* FlipBasis.jl:77 and the like */
+      free(b);
+      return NULL;
+    }
+    b->P = p;
+    b->Z = z;
+    if (kind == ORC_TRANSLATION_FLIP) {
+      b->kp = ((-k) % b->len + b->len) % b->len; /* TranslationalFlipBasis.jl:136 */
+      b->C = (cplx *)malloc(b->len * sizeof(cplx));
+      for (int i = 0; i < b->len; i++) { /* phase_factor (TranslationalBasis.jl:277-285) */
+        if (b->kp == 0) b->C[i] = 1.0;
+        else if (2 * b->kp == b->len) b->C[i] = (i % 2 == 0) ? 1.0 : -1.0;
+        else {
+          double th = ((-2.0 * M_PI / b->len) * b->kp) * i;
+          b->C[i] = cos(th) + IU * sin(th);
+        }
+      }
+    }
   } else if (kind != ORC_PROJECTED) {
     set_err("unknown basis kind");
     free(b);
@@ -670,6 +779,87 @@ static inline void basis_index(const orc_basis *b, uint64_t s, cplx *coef, int64
           R = i;
         }
         uint64_t Ib1 = reverse_digits(b, st) + 1;
+        if (Ib1 < Ibm) {
+          Ibm = Ib1;
+          M = i;
+        }
+      }
+      int64_t i;
+      cplx n;
+      if (Ibm < Iam) {
+        i = bsearch1(b->tab, b->D, (int64_t)Ibm);
+        n = (double)b->P * b->C[M];
+      } else {
+        i = bsearch1(b->tab, b->D, (int64_t)Iam);
+        n = b->C[R];
+      }
+      if (i == 0) {
+        *coef = 0.0;
+        *pos = 1;
+        return;
+      }
+      *coef = n * b->R[i - 1];
+      *pos = i;
+      return;
+    }
+    case ORC_PARITY:   /* ParityBasis.jl:110-121 */
+    case ORC_FLIP: {   /* FlipBasis.jl:105-115 */
+      uint64_t Ia = s + 1;
+      uint64_t Ib = ((b->kind == ORC_PARITY) ? reverse_digits(b, s) : spinflip(b, s)) + 1;
+      int64_t i;
+      double n;
+      if (Ib < Ia) {
+        i = bsearch1(b->tab, b->D, (int64_t)Ib);
+        n = (double)b->P;
+      } else {
+        i = bsearch1(b->tab, b->D, (int64_t)Ia);
+        n = 1.0;
+      }
+      if (i == 0) {
+        *coef = 0.0;
+        *pos = 1;
+        return;
+      }
+      *coef = n * b->R[i - 1];
+      *pos = i;
+      return;
+    }
+    case ORC_PARITY_FLIP: { /* ParityFlipBasis.jl:122-145 */
+      uint64_t I0 = s + 1, Ip = reverse_digits(b, s) + 1, Iz = spinflip(b, s) + 1;
+      uint64_t Ipz = spinflip(b, Ip - 1) + 1;
+      uint64_t Ir = I0;
+      if (Iz < Ir) Ir = Iz;
+      if (Ip < Ir) Ir = Ip;
+      if (Ipz < Ir) Ir = Ipz;
+      int64_t i = bsearch1(b->tab, b->D, (int64_t)Ir);
+      if (i == 0) {
+        *coef = 0.0;
+        *pos = 1;
+        return;
+      }
+      double n;
+      if (Ir == I0) n = 1.0;
+      else if (Ir == Ip) n = (double)b->P;
+      else if (Ir == Iz) n = (double)b->Z;
+      else n = (double)(b->P * b->Z);
+      *coef = n * b->R[i - 1];
+      *pos = i;
+      return;
+    }
+    case ORC_TRANSLATION_FLIP: { /* TranslationalFlipBasis.jl:172-215 */
+      uint64_t Ia0 = s + 1, st = s;
+      uint64_t Ib0 = spinflip(b, s) + 1;
+      uint64_t Iam = Ia0, Ibm = Ib0;
+      int R = 0, M = 0;
+      for (int i = 1; i <= b->len - 1; i++) {
+        st = translate(b, st);
+        uint64_t Ia1 = st + 1;
+        if (Ia1 == Ia0) break;
+        else if (Ia1 < Iam) {
+          Iam = Ia1;
+          R = i;
+        }
+        uint64_t Ib1 = spinflip(b, st) + 1;
         if (Ib1 < Ibm) {
           Ibm = Ib1;
           M = i;

@@ -33,7 +33,11 @@ enum {
   ORC_PROJECTED = 1,
   ORC_TRANSLATIONAL = 2,
   ORC_TRANSLATION_PARITY = 3,
-  ORC_ABELIAN = 4
+  ORC_ABELIAN = 4,
+  ORC_PARITY = 5,            /* ParityBasis.jl */
+  ORC_FLIP = 6,              /* FlipBasis.jl */
+  ORC_PARITY_FLIP = 7,       /* ParityFlipBasis.jl */
+  ORC_TRANSLATION_FLIP = 8   /* TranslationalFlipBasis.jl */
 };
 
 typedef struct orc_basis orc_basis;

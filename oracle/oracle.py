@@ -13,7 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 KIND = {"tensor": 0, "projected": 1, "translational": 2,
-        "translation_parity": 3, "abelian": 4}
+        "translation_parity": 3, "abelian": 4, "parity": 5, "flip": 6, "parity_flip": 7,
+        "translation_flip": 8}
 
 
 def lib():

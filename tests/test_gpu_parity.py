@@ -12,7 +12,9 @@ from tests.models import blbq, hopping_pm, j1j2, rand_state, xxz
 pytestmark = pytest.mark.gpu
 
 OKIND = {"TensorBasis": "tensor", "ProjectedBasis": "projected",
-         "TranslationalBasis": "translational", "TranslationParityBasis": "translation_parity"}
+         "TranslationalBasis": "translational", "TranslationParityBasis": "translation_parity",
+         "ParityBasis": "parity", "FlipBasis": "flip", "ParityFlipBasis": "parity_flip",
+         "TranslationFlipBasis": "translation_flip"}
 
 
 def make_pair(edk, kind, L, base=2, **kw):
@@ -49,6 +51,22 @@ TABLE_CASES = [
     ("basis", 10, 2, dict(k=1, p=1)),
     ("basis", 8, 3, dict(N=8, k=1)),
     ("basis", 12, 2, dict(N=5, k=0)),
+    ("ParityBasis", 12, 2, dict(N=6, p=1)),
+    ("ParityBasis", 11, 2, dict(p=-1)),
+    ("ParityBasis", 7, 3, dict(N=7, p=-1)),
+    ("FlipBasis", 12, 2, dict(N=6, p=-1)),
+    ("FlipBasis", 11, 2, dict(p=1)),
+    ("FlipBasis", 7, 3, dict(N=7, p=1)),
+    ("ParityFlipBasis", 12, 2, dict(N=6, p=1, z=1)),
+    ("ParityFlipBasis", 12, 2, dict(N=6, p=-1, z=1)),
+    ("ParityFlipBasis", 10, 2, dict(p=1, z=-1)),
+    ("ParityFlipBasis", 7, 3, dict(p=-1, z=-1)),
+    ("TranslationFlipBasis", 12, 2, dict(N=6, k=0, p=1)),
+    ("TranslationFlipBasis", 12, 2, dict(N=6, k=3, p=-1)),
+    ("TranslationFlipBasis", 12, 2, dict(N=6, k=6, p=1)),
+    ("TranslationFlipBasis", 12, 2, dict(N=6, k=1, p=1, a=2)),
+    ("TranslationFlipBasis", 11, 2, dict(k=4, p=-1)),
+    ("TranslationFlipBasis", 8, 3, dict(N=8, k=2, p=1)),
 ]
 
 
@@ -103,6 +121,13 @@ MV_CASES = [
     ("basis", 14, 2, dict(N=7, k=0, p=-1), j1j2),
     ("basis", 12, 2, dict(N=6, k=4, z=1), xxz),
     ("basis", 12, 2, dict(N=6, k=5), j1j2),
+    ("ParityBasis", 16, 2, dict(N=8, p=-1), j1j2),
+    ("FlipBasis", 16, 2, dict(N=8, p=1), xxz),
+    ("ParityFlipBasis", 16, 2, dict(N=8, p=-1, z=-1), j1j2),
+    ("TranslationFlipBasis", 16, 2, dict(N=8, k=5, p=-1), j1j2),
+    ("TranslationFlipBasis", 16, 2, dict(N=8, k=8, p=1), hopping_pm),
+    ("FlipBasis", 9, 3, dict(N=9, p=-1), blbq),
+    ("TranslationFlipBasis", 8, 3, dict(N=8, k=3, p=1), blbq),
 ]
 
 
@@ -131,6 +156,31 @@ def test_matvec_c32_matches_oracle(gpu, kind, L, base, kw, model):
     assert y.dtype == np.complex64
     yo = orc.Op(ob, H.flatten()) @ x.astype(np.complex64).astype(np.complex128)
     assert np.linalg.norm(y - yo) <= 2e-6 * np.linalg.norm(yo)
+
+
+@pytest.mark.parametrize("kind,L,kw,model", [
+    ("ParityFlipBasis", 14, dict(N=7, p=1, z=-1), xxz),
+    ("TranslationFlipBasis", 14, dict(N=7, k=3, p=1), j1j2),
+])
+def test_flip_bases_blocks_and_assembly(gpu, kind, L, kw, model):
+    """Blocks of vectors (one canonical form per target for all columns) and
+    the assembled sector matrix of the flip bases against the oracle."""
+    import edkit_hip as edk
+    B, ob = make_pair(edk, kind, L, **kw)
+    H = model(B)
+    op = orc.Op(ob, H.flatten())
+    X = np.stack([rand_state(B.D, 30 + c) for c in range(5)], axis=1)
+    for dt, tol in ((np.complex128, 1e-12), (np.complex64, 2e-6)):
+        Y = H @ X.astype(dt)
+        for c in range(5):
+            yo = op @ X[:, c].astype(dt).astype(np.complex128)
+            assert np.linalg.norm(Y[:, c] - yo) <= tol * np.linalg.norm(yo), (kind, dt, c)
+    S = edk.sparse(H)
+    for j in (0, B.D // 3, B.D - 1):
+        e = np.zeros(B.D, np.complex128)
+        e[j] = 1
+        col = op @ e
+        assert np.linalg.norm(S[:, [j]].toarray().ravel() - col) <= 1e-12 * max(np.linalg.norm(col), 1)
 
 
 def test_matvec_block_and_device_vectors(gpu):

@@ -140,6 +140,62 @@ def test_base3_tables_by_definition():
                   lambda d: sum(d) == 6)
 
 
+@pytest.mark.parametrize("L", [6, 7, 8])
+def test_parity_and_flip_tables_by_definition(L):
+    """ParityBasis / FlipBasis / ParityFlipBasis / TranslationFlipBasis tables
+    against orbit minima with a nonzero projected vector, and their norms
+    against the projected vectors' norms (palindromes and self-flipped states
+    kept only at eigenvalue +1)."""
+    everything = lambda d: True  # noqa: E731
+    half = lambda d: 2 * sum(d) == L  # noqa: E731
+    for p in (1, -1):
+        _check_tables(orc.Basis("parity", L, p=p), L, 2, [dense.parity_gen(L, p)], everything)
+        _check_tables(orc.Basis("flip", L, p=p), L, 2, [dense.flip_gen(L, p)], everything)
+        for z in (1, -1):
+            _check_tables(orc.Basis("parity_flip", L, p=p, z=z), L, 2,
+                          [dense.parity_gen(L, p), dense.flip_gen(L, z)], everything)
+        if L % 2 == 0:
+            N = L // 2
+            _check_tables(orc.Basis("parity", L, N=N - 1, p=p), L, 2, [dense.parity_gen(L, p)],
+                          lambda d: sum(d) == L - N + 1)
+            _check_tables(orc.Basis("flip", L, N=N, p=p), L, 2, [dense.flip_gen(L, p)], half)
+            for k in range(L):
+                _check_tables(orc.Basis("translation_flip", L, N=N, k=k, p=p), L, 2,
+                              [dense.translation_gens(L, k), dense.flip_gen(L, p)], half)
+        for k in range(L):
+            _check_tables(orc.Basis("translation_flip", L, k=k, p=p), L, 2,
+                          [dense.translation_gens(L, k), dense.flip_gen(L, p)], everything)
+
+
+def test_flip_bases_base3_and_unit_cell():
+    L = 4
+    for p in (1, -1):
+        _check_tables(orc.Basis("parity", L, base=3, p=p), L, 3, [dense.parity_gen(L, p)], lambda d: True)
+        _check_tables(orc.Basis("flip", L, base=3, N=L, p=p), L, 3, [dense.flip_gen(L, p)],
+                      lambda d: sum(d) == L)
+        for z in (1, -1):
+            _check_tables(orc.Basis("parity_flip", L, base=3, p=p, z=z), L, 3,
+                          [dense.parity_gen(L, p), dense.flip_gen(L, z)], lambda d: True)
+        for k in range(L):
+            _check_tables(orc.Basis("translation_flip", L, base=3, k=k, p=p), L, 3,
+                          [dense.translation_gens(L, k), dense.flip_gen(L, p)], lambda d: True)
+    L = 8
+    for k in range(4):  # unit cell of two sites
+        for p in (1, -1):
+            _check_tables(orc.Basis("translation_flip", L, N=4, k=k, p=p, a=2), L, 2,
+                          [dense.translation_gens(L, k, a=2), dense.flip_gen(L, p)],
+                          lambda d: sum(d) == 4)
+
+
+def test_flip_bases_reject_incompatible_arguments():
+    with pytest.raises(Exception):
+        orc.Basis("flip", 8, N=3, p=1)
+    with pytest.raises(Exception):
+        orc.Basis("parity", 8, p=2)
+    with pytest.raises(Exception):
+        orc.Basis("parity_flip", 8, p=1, z=0)
+
+
 def test_translational_and_abelian_tables_agree():
     # basis(L, k=0).I == TranslationalBasis(L, k=0).I (abelian_overhaul_tests.jl:303-313)
     for L in (8, 10):
@@ -173,6 +229,18 @@ CASES = [
     ("abelian", dict(N=4, k=1, z=1, has_k=True, has_z=True), 8, 2, xxz),
     ("translational", dict(N=6, k=1), 6, 3, blbq),
     ("translation_parity", dict(N=6, k=0, p=1), 6, 3, blbq),
+    ("parity", dict(N=4, p=-1), 8, 2, j1j2),
+    ("parity", dict(p=1), 7, 2, xxz),
+    ("flip", dict(N=4, p=1), 8, 2, xxz),
+    ("flip", dict(p=-1), 7, 2, hopping_pm),
+    ("parity_flip", dict(N=4, p=1, z=-1), 8, 2, j1j2),
+    ("parity_flip", dict(N=4, p=-1, z=-1), 8, 2, xxz),
+    ("translation_flip", dict(N=4, k=0, p=1), 8, 2, xxz),
+    ("translation_flip", dict(N=4, k=3, p=-1), 8, 2, j1j2),
+    ("translation_flip", dict(N=4, k=1, p=1, a=2), 8, 2, xxz),
+    ("flip", dict(N=6, p=-1), 6, 3, blbq),
+    ("parity_flip", dict(N=6, p=1, z=1), 6, 3, blbq),
+    ("translation_flip", dict(N=6, k=2, p=-1), 6, 3, blbq),
 ]
 
 
@@ -200,6 +268,28 @@ def test_sector_matrix_is_compression_of_full_operator(kind, kw, L, base, model)
     np.testing.assert_allclose(Q.conj().T @ O @ Q, Hs, atol=1e-12)
     np.testing.assert_allclose(O @ Q, Q @ Hs, atol=1e-12)
     np.testing.assert_allclose(Hs, Hs.conj().T, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind,labels", [
+    ("parity", [dict(p=1), dict(p=-1)]),
+    ("flip", [dict(p=1), dict(p=-1)]),
+    ("parity_flip", [dict(p=p, z=z) for p in (1, -1) for z in (1, -1)]),
+    ("translation_flip", [dict(k=k, p=p) for k in range(8) for p in (1, -1)]),
+])
+def test_spectrum_union_over_flip_sectors(kind, labels):
+    """The sectors of every eigenvalue label reassemble the spectrum of the
+    half-filled charge sector (nothing lost, nothing doubled)."""
+    L, N = 8, 4
+    H = j1j2(Shape(L), 0.35)
+    O = dense.full_operator(H, L)
+    I, _ = orc.Basis("projected", L, N=N).tables()
+    full = np.sort(np.linalg.eigvalsh(O[np.ix_(I - 1, I - 1)]))
+    vals = []
+    for kw in labels:
+        ob = orc.Basis(kind, L, N=N, **kw)
+        if ob.D:
+            vals += list(np.linalg.eigvalsh(_sector_matrix(orc.Op(ob, H.flatten()), ob.D)))
+    np.testing.assert_allclose(np.sort(vals), full, atol=1e-10)
 
 
 def test_spectrum_union_over_momenta():
