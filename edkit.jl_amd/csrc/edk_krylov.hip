@@ -399,6 +399,90 @@ int edk_tridiag_eig(int n, const double *alpha, const double *beta, double *d, d
   return EDK_OK;
 }
 
+// Lowest eigenvalue of the symmetric tridiagonal (alpha, beta) and the
+// magnitude of the last component of its normalised eigenvector, in O(n):
+// bisection on Sturm counts (Golub & Van Loan 8.4.1) to full precision, then
+// two steps of inverse iteration on T - theta I with a row-interchanging LU
+// (the method of LAPACK's dstein).  The per-step convergence test of the
+// Lanczos ground state needs only these two numbers; the QL solver above is
+// O(n^2) per call.
+static int sturm_below(int n, const double *a, const double *b, double x) {
+  int c = 0;
+  double d = 1.0;
+  for (int k = 0; k < n; k++) {
+    d = (a[k] - x) - (k ? b[k - 1] * b[k - 1] / d : 0.0);
+    if (d == 0.0) d = -1e-300;
+    if (d < 0) c++;
+  }
+  return c;
+}
+
+static void lowest_ritz(int n, const double *a, const double *b, double *theta, double *qlast) {
+  double lo = INFINITY, hi = -INFINITY, tn = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double r = (k ? std::fabs(b[k - 1]) : 0.0) + (k < n - 1 ? std::fabs(b[k]) : 0.0);
+    lo = std::min(lo, a[k] - r);
+    hi = std::max(hi, a[k] + r);
+    tn = std::max(tn, std::fabs(a[k]) + r);
+  }
+  for (int it = 0; it < 256; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_below(n, a, b, mid) >= 1) hi = mid;
+    else lo = mid;
+  }
+  const double th = hi;
+  *theta = th;
+  // LU with row interchanges of A = T - th I: U has diagonal d, superdiagonals du, du2
+  std::vector<double> d(n), du(n, 0.0), du2(n, 0.0), dl(n, 0.0), x(n, 1.0);
+  std::vector<char> piv(n, 0);
+  const double pert = 2.220446049250313e-16 * std::max(tn, 1e-300);
+  for (int k = 0; k < n; k++) {
+    d[k] = a[k] - th;
+    if (k < n - 1) du[k] = dl[k] = b[k];
+  }
+  for (int k = 0; k + 1 < n; k++) {
+    if (std::fabs(d[k]) >= std::fabs(dl[k])) {
+      if (d[k] == 0.0) d[k] = pert;
+      const double l = dl[k] / d[k];
+      dl[k] = l;
+      d[k + 1] -= l * du[k];
+    } else {
+      const double l = d[k] / dl[k];
+      d[k] = dl[k];
+      dl[k] = l;
+      const double t = du[k];
+      du[k] = d[k + 1];
+      d[k + 1] = t - l * d[k + 1];
+      if (k + 2 < n) {
+        du2[k] = du[k + 1];
+        du[k + 1] = -l * du2[k];
+      }
+      piv[k] = 1;
+    }
+  }
+  if (d[n - 1] == 0.0) d[n - 1] = pert;
+  for (int it = 0; it < 2; it++) {
+    for (int k = 0; k + 1 < n; k++) {
+      if (!piv[k]) x[k + 1] -= dl[k] * x[k];
+      else {
+        const double t = x[k];
+        x[k] = x[k + 1];
+        x[k + 1] = t - dl[k] * x[k];
+      }
+    }
+    x[n - 1] /= d[n - 1];
+    if (n > 1) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / d[n - 2];
+    for (int k = n - 3; k >= 0; k--) x[k] = (x[k] - du[k] * x[k + 1] - du2[k] * x[k + 2]) / d[k];
+    double mx = 0.0;
+    for (int k = 0; k < n; k++) mx = std::max(mx, std::fabs(x[k]));
+    for (int k = 0; k < n; k++) x[k] /= mx;
+  }
+  double s2 = 0.0;
+  for (int k = 0; k < n; k++) s2 += x[k] * x[k];
+  *qlast = std::fabs(x[n - 1]) / std::sqrt(s2);
+}
+
 // ---------------------------------------------------------------------------
 // Lanczos step: the _lanczos_build! recurrence (TimeEvolution.jl:250-306)
 // ---------------------------------------------------------------------------
@@ -545,12 +629,22 @@ int edk_lanczos_run(edk_op *op, edk_ctx *ctx, int dtype, const void *x0, int m, 
       rc = -br;
       break;
     }
-    // convergence needs only the last row of the eigenvector matrix
+    if (br) break;
+    if (tol > 0) {
+      // residual estimate |beta_m q_m| of the lowest Ritz pair, O(m) per step
+      double th, qm;
+      lowest_ritz(mm, alpha.data(), beta.data(), &th, &qm);
+      if (std::fabs(beta[mm - 1] * qm) <= tol) break;
+    }
+  }
+  if (!rc && mm > 0) {
+    // the reported pair: the QL solver once, at the final step
     const int last = mm - 1;
-    if ((rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data(), 1, &last))) break;
-    e0 = lam[0];
-    res = br ? 0.0 : std::fabs(beta[mm - 1] * Q[0]);
-    if (br || (tol > 0 && res <= tol)) break;
+    rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data(), 1, &last);
+    if (!rc) {
+      e0 = lam[0];
+      res = beta[mm - 1] == 0.0 ? 0.0 : std::fabs(beta[mm - 1] * Q[0]);
+    }
   }
   if (!rc && (refine || vec_out)) rc = edk_tridiag_eig(mm, alpha.data(), beta.data(), lam.data(), Q.data(), mm, nullptr);
   if (!rc && (refine || vec_out)) {
