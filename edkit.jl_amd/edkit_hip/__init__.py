@@ -5,11 +5,12 @@ The names mirror the Julia API: bases (TensorBasis, ProjectedBasis,
 TranslationalBasis, TranslationParityBasis, ParityBasis, FlipBasis,
 ParityFlipBasis, TranslationFlipBasis, basis), spin, operator,
 trans_inv_operator, mul_ (mul!), H @ x / H * x, mul, timeevolve,
-KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, plus the new
-lanczos_ground.
+KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, schmidt,
+ent_spec, ent_S, plus the new lanczos_ground.
 """
 from ._lib import EdkError, lib  # noqa: F401
 from .batch import MatvecBatch  # noqa: F401
+from .schmidt import ent_S, ent_spec, entropy, schmidt  # noqa: F401
 from .basis import (AbelianBasis, AbstractBasis, FlipBasis, ParityBasis,  # noqa: F401
                     ParityFlipBasis, ProjectedBasis, TensorBasis, TranslationalBasis,
                     TranslationFlipBasis, TranslationParityBasis, basis, productstate)

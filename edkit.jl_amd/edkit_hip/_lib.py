@@ -35,6 +35,7 @@ SYMBOLS = [
     "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2",
     "edk_lanczos_ground",
     "edk_lanczos_ground_ex",
+    "edk_schmidt", "edk_ent_spec",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
     "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
 ]
@@ -160,6 +161,8 @@ def lib():
                                      C.POINTER(C.c_int), P]),
         "edk_lanczos_ground_ex": (I32, [P, I32, P, I32, D, I32, I32, C.POINTER(D), C.POINTER(D),
                                         C.POINTER(C.c_int), C.POINTER(C.c_int), P]),
+        "edk_schmidt": (I32, [P, I32, P, I32, P, P]),
+        "edk_ent_spec": (I32, [P, I32, P, I32, P, P, C.POINTER(I64)]),
         "edk_krylov_default_params": (None, [C.POINTER(KrylovParams)]),
         "edk_krylov_create": (I32, [P, I32, P, C.POINTER(KrylovParams), PP]),
         "edk_krylov_evolve": (I32, [P, D, P]),

@@ -227,6 +227,20 @@ int edk_vec_dot(edk_ctx *ctx, int dtype, const void *x, const void *y, int64_t n
                 double *im);                                                              /* x^H y */
 int edk_vec_nrm2(edk_ctx *ctx, int dtype, const void *x, int64_t n, double *out);         /* ||x|| */
 
+/* ---- Schmidt decomposition of a sector state ------------------------------- */
+/* schmidt(v, Ainds, b) (Schmidt.jl:186-330, AbelianBasis.jl:704-719): the
+ * dense Schmidt matrix of the device vector v (D elements of dtype) for the
+ * subsystem of the nA 1-based sites Ainds, written to the device buffer M of
+ * base^nA x base^(L-nA) ComplexF64 entries, column-major (M[ia + dA ib], ia and
+ * ib the 0-based indices of the A and B digits; the B sites in increasing
+ * order).  Needs a device-built basis (or a tensor/projected one). */
+int edk_schmidt(edk_basis *b, int dtype, const void *v, int nA, const int32_t *Ainds, void *M);
+/* The squared Schmidt values of the same matrix (svdvals(schmidt(...)) .^ 2,
+ * the argument of entropy in ent_S, Schmidt.jl:150-154) in descending order:
+ * s2 receives min(dA, dB) values (host array), *n their number. */
+int edk_ent_spec(edk_basis *b, int dtype, const void *v, int nA, const int32_t *Ainds, double *s2,
+                 int64_t *n);
+
 /* ---- Krylov --------------------------------------------------------------- */
 /* Lanczos ground state (no counterpart in the reference; built from the
  * _lanczos_build! recurrence TimeEvolution.jl:250-306 with full

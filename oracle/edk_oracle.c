@@ -907,6 +907,101 @@ static inline void basis_index(const orc_basis *b, uint64_t s, cplx *coef, int64
   }
 }
 
+/* ---- schmidt (Schmidt.jl:186-330, AbelianBasis.jl:704-719) ---- */
+/* addto! (Schmidt.jl:65-72): M[index(A digits), index(B digits)] += val, the
+ * digits of site q (1-based) being the q-th most significant of the content */
+static void schmidt_add(const orc_basis *b, const int *Ainds, int nA, const int *Binds, int nB, int64_t dA,
+                        cplx *M, uint64_t s, cplx val) {
+  int d[MAXL];
+  uint64_t v = s;
+  for (int i = b->L - 1; i >= 0; i--) {
+    d[i] = (int)(v % b->base);
+    v /= b->base;
+  }
+  int64_t ia = 0, ib = 0;
+  for (int q = 0; q < nA; q++) ia = ia * b->base + d[Ainds[q] - 1];
+  for (int q = 0; q < nB; q++) ib = ib * b->base + d[Binds[q] - 1];
+  M[ia + dA * ib] += val;
+}
+
+/* M: base^nA x base^(L-nA), column-major, zeroed by the caller */
+void orc_schmidt(const orc_basis *b, const double *v, int nA, const int32_t *Ainds, double *Mout) {
+  cplx *M = (cplx *)Mout;
+  const cplx *x = (const cplx *)v;
+  int A[MAXL], B[MAXL], nB = 0, in[MAXL + 1] = {0};
+  for (int q = 0; q < nA; q++) {
+    A[q] = Ainds[q];
+    in[Ainds[q]] = 1;
+  }
+  for (int q = 1; q <= b->L; q++) /* Binds (Schmidt.jl:46-52) */
+    if (!in[q]) B[nB++] = q;
+  int64_t dA = 1;
+  for (int q = 0; q < nA; q++) dA *= b->base;
+  if (b->kind == ORC_TENSOR || b->kind == ORC_PROJECTED) { /* Schmidt.jl:186-195 */
+    for (int64_t i = 0; i < b->D; i++) {
+      uint64_t s = b->kind == ORC_TENSOR ? (uint64_t)i : (uint64_t)(b->tab[i] - 1);
+      schmidt_add(b, A, nA, B, nB, dA, M, s, x[i]);
+    }
+    return;
+  }
+  /* b.C[2] */
+  cplx step = 1.0;
+  if ((b->kind == ORC_TRANSLATIONAL || b->kind == ORC_TRANSLATION_PARITY || b->kind == ORC_TRANSLATION_FLIP) &&
+      b->len > 1)
+    step = b->C[1];
+  for (int64_t i = 0; i < b->D; i++) {
+    uint64_t s = (uint64_t)(b->tab[i] - 1);
+    cplx val = x[i] / b->R[i];
+    switch (b->kind) {
+      case ORC_TRANSLATIONAL:       /* Schmidt.jl:202-216 */
+      case ORC_TRANSLATION_PARITY:  /* parity_schmidt :249-273 */
+      case ORC_TRANSLATION_FLIP:
+        for (int j = 0; j < b->len; j++) {
+          schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+          s = translate(b, s);
+          val *= step;
+        }
+        if (b->kind == ORC_TRANSLATIONAL) break;
+        s = b->kind == ORC_TRANSLATION_PARITY ? reverse_digits(b, s) : spinflip(b, s);
+        val *= (double)b->P;
+        for (int j = 0; j < b->len; j++) {
+          schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+          s = translate(b, s);
+          val *= step;
+        }
+        break;
+      case ORC_PARITY: /* Schmidt.jl:296-309 */
+      case ORC_FLIP:   /* Schmidt.jl:279-292 */
+        schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+        s = b->kind == ORC_PARITY ? reverse_digits(b, s) : spinflip(b, s);
+        schmidt_add(b, A, nA, B, nB, dA, M, s, (double)b->P * val);
+        break;
+      case ORC_PARITY_FLIP: /* Schmidt.jl:313-330 */
+        schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+        s = reverse_digits(b, s);
+        val *= (double)b->P;
+        schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+        s = spinflip(b, s);
+        val *= (double)b->Z;
+        schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+        s = reverse_digits(b, s);
+        val *= (double)b->P;
+        schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+        break;
+      case ORC_ABELIAN: { /* AbelianBasis.jl:704-719 */
+        int os[MAXGEN];
+        for (int g = 0; g < b->ngen; g++) os[g] = 1;
+        schmidt_add(b, A, nA, B, nB, dA, M, s, val);
+        for (int st = 2; st <= b->order; st++) {
+          s = odometer_step(b, os, s);
+          schmidt_add(b, A, nA, B, nB, dA, M, s, abelian_phase(b, os) * val);
+        }
+        break;
+      }
+    }
+  }
+}
+
 void orc_basis_index(const orc_basis *b, const uint64_t *states, int64_t n,
                      int64_t *pos1, double *cre, double *cim) {
   for (int64_t q = 0; q < n; q++) {

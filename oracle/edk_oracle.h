@@ -62,6 +62,10 @@ int64_t orc_basis_dim(const orc_basis *b);
 /* I (1-based contents, exactly as the reference stores b.I) and R. */
 void orc_basis_tables(const orc_basis *b, int64_t *I1, double *R);
 /* index(b, dgt) for n packed 0-based states: 1-based pos and complex coef. */
+/* schmidt(v, Ainds, b) (Schmidt.jl:186-330, AbelianBasis.jl:704-719): v is
+ * D complex doubles, M the zeroed base^nA x base^(L-nA) complex matrix,
+ * column-major; Ainds 1-based. */
+void orc_schmidt(const orc_basis *b, const double *v, int nA, const int32_t *Ainds, double *M);
 void orc_basis_index(const orc_basis *b, const uint64_t *states, int64_t n,
                      int64_t *pos1, double *coef_re, double *coef_im);
 int orc_basis_order(const orc_basis *b);

@@ -39,6 +39,7 @@ def lib():
         L.orc_basis_order.argtypes = [P]
         L.orc_basis_tables.argtypes = [P, i64p, f64p]
         L.orc_basis_index.argtypes = [P, u64p, C.c_int64, i64p, f64p, f64p]
+        L.orc_schmidt.argtypes = [P, f64p, C.c_int, i32p, f64p]
         L.orc_op_create.restype = P
         L.orc_op_create.argtypes = [P, C.c_int64, i64p, i64p, i64p, i64p, i64p, i64p, f64p]
         L.orc_op_free.argtypes = [P]
@@ -136,6 +137,15 @@ class Basis:
         cim = np.zeros(n)
         lib().orc_basis_index(self.h, states, n, pos, cre, cim)
         return pos, cre + 1j * cim
+
+    def schmidt(self, v, Ainds):
+        """schmidt(v, Ainds, b) (Schmidt.jl:186-330): the dA x dB matrix."""
+        x = np.ascontiguousarray(np.asarray(v, np.complex128))
+        A = np.ascontiguousarray(np.asarray(list(Ainds), np.int32))
+        dA, dB = self.base ** len(A), self.base ** (self.L - len(A))
+        M = np.zeros(dA * dB, np.complex128)
+        lib().orc_schmidt(self.h, x.view(np.float64), len(A), A, M.view(np.float64))
+        return M.reshape((dB, dA)).T
 
 
 class Op:

@@ -460,3 +460,47 @@ def test_krylov_errors():
         c.evolve(0.5)
     with pytest.raises(ValueError):
         orc.timeevolve(op, rand_state(16, 1), [-1.0])
+
+
+# ------------------------------------------------------------- schmidt / ent_S
+SCHMIDT_CASES = [
+    ("tensor", dict(), 7, 2),
+    ("projected", dict(N=4), 8, 2),
+    ("translational", dict(N=4, k=3), 8, 2),
+    ("translational", dict(N=4, k=1, a=2), 8, 2),
+    ("translation_parity", dict(N=4, k=4, p=-1), 8, 2),
+    ("parity", dict(N=4, p=-1), 8, 2),
+    ("flip", dict(N=4, p=1), 8, 2),
+    ("parity_flip", dict(N=4, p=-1, z=1), 8, 2),
+    ("translation_flip", dict(N=4, k=3, p=-1), 8, 2),
+    ("abelian", dict(N=4, k=2, has_k=True, z=-1, has_z=True), 8, 2),
+    ("translational", dict(N=6, k=1), 6, 3),
+]
+
+
+@pytest.mark.parametrize("kind,kw,L,base", SCHMIDT_CASES)
+def test_schmidt_matrix_is_the_embedded_state(kind, kw, L, base):
+    """schmidt(v, Ainds, b) element by element against the full-space state
+    Q v built from index() (the embedding of the compression test above),
+    reshaped by the A and B digits; a unit vector gives a unit matrix."""
+    ob = orc.Basis(kind, L, base=base, **kw)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=ob.D) + 1j * rng.normal(size=ob.D)
+    v /= np.linalg.norm(v)
+    A = [1, 2, 4, L - 1]
+    M = ob.schmidt(v, A)
+    n = base ** L
+    pos, coef = ob.index(np.arange(n, dtype=np.uint64))
+    gsize = ob.order // kw.get("a", 1) if kind != "abelian" else ob.order
+    gsize = 1 if kind in ("tensor", "projected") else gsize
+    P = np.zeros((n, ob.D), np.complex128)
+    nz = coef != 0
+    P[np.nonzero(nz)[0], pos[nz] - 1] = coef[nz] / gsize
+    psi = P.conj() @ v
+    Bs = [q for q in range(1, L + 1) if q not in A]
+    Md = np.zeros_like(M)
+    for c in range(n):
+        d = dense.digits(c, L, base)
+        Md[dense.content([d[q - 1] for q in A], base), dense.content([d[q - 1] for q in Bs], base)] = psi[c]
+    np.testing.assert_allclose(M, Md, atol=1e-14)
+    assert abs(np.linalg.norm(M) - 1) < 1e-12
