@@ -6,7 +6,9 @@ TranslationalBasis, TranslationParityBasis, ParityBasis, FlipBasis,
 ParityFlipBasis, TranslationFlipBasis, basis), spin, operator,
 trans_inv_operator, mul_ (mul!), H @ x / H * x, mul, timeevolve,
 KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, schmidt,
-ent_spec, ent_S, plus the new lanczos_ground.
+ent_spec, ent_S, lindblad, densitymatrix, LiouvillianMap,
+LindbladArnoldiCache, lindblad_timeevolve, lindblad_timeevolve_
+(lindblad_timeevolve!), plus the new lanczos_ground.
 """
 from ._lib import EdkError, lib  # noqa: F401
 from .batch import MatvecBatch  # noqa: F401
@@ -15,6 +17,8 @@ from .basis import (AbelianBasis, AbstractBasis, FlipBasis, ParityBasis,  # noqa
                     ParityFlipBasis, ProjectedBasis, TensorBasis, TranslationalBasis,
                     TranslationFlipBasis, TranslationParityBasis, basis, productstate)
 from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401
+from .lindblad import (DensityMatrix, Lindblad, LindbladArnoldiCache, LiouvillianMap,  # noqa: F401
+                       densitymatrix, lindblad, lindblad_timeevolve, lindblad_timeevolve_)
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
 from .matvec import (Array, apply, clear_sparse_cache_, device_operator, matvec_into, mul,  # noqa: F401
                      mul_, sparse, sparse_)

@@ -38,6 +38,9 @@ SYMBOLS = [
     "edk_schmidt", "edk_ent_spec",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
     "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
+    "edk_liouvillian_create", "edk_liouvillian_apply", "edk_liouvillian_destroy",
+    "edk_lindblad_default_params", "edk_lindblad_create", "edk_lindblad_evolve",
+    "edk_lindblad_diagnostics", "edk_lindblad_records", "edk_lindblad_dim", "edk_lindblad_destroy",
 ]
 
 
@@ -98,6 +101,20 @@ class KrylovDiag(C.Structure):
                 ("restarts", C.c_int64), ("matvecs", C.c_int64),
                 ("total_times_served", C.c_int64), ("max_dim_used", C.c_int64),
                 ("n_accepted_intervals", C.c_int64)]
+
+
+class LindbladParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("m_init", C.c_int32), ("m_max", C.c_int32),
+                ("extend_step", C.c_int32), ("nsample", C.c_int32), ("bisect_iters", C.c_int32),
+                ("extend_basis", C.c_int32), ("normalize_trace", C.c_int32), ("record_min_eig", C.c_int32)]
+
+
+class LindbladDiag(C.Structure):
+    _fields_ = [("basis_builds", C.c_int64), ("basis_extensions", C.c_int64),
+                ("restarts", C.c_int64), ("liouvillian_applies", C.c_int64),
+                ("total_times_served", C.c_int64), ("max_dim_used", C.c_int64),
+                ("n_accepted_intervals", C.c_int64), ("max_trace_drift", C.c_double),
+                ("max_hermiticity_drift", C.c_double), ("min_min_hermitian_eig", C.c_double)]
 
 
 _LIB = None
@@ -169,6 +186,17 @@ def lib():
         "edk_krylov_diagnostics": (I32, [P, C.POINTER(KrylovDiag)]),
         "edk_krylov_accepted_intervals": (I32, [P, P, I64, C.POINTER(C.c_int64)]),
         "edk_krylov_destroy": (I32, [P]),
+        "edk_liouvillian_create": (I32, [P, I64, P, I32, P, PP]),
+        "edk_liouvillian_apply": (I32, [P, P, P]),
+        "edk_liouvillian_destroy": (I32, [P]),
+        "edk_lindblad_default_params": (None, [C.POINTER(LindbladParams)]),
+        "edk_lindblad_create": (I32, [P, P, C.POINTER(LindbladParams), PP]),
+        "edk_lindblad_evolve": (I32, [P, D, P]),
+        "edk_lindblad_diagnostics": (I32, [P, C.POINTER(LindbladDiag)]),
+        "edk_lindblad_records": (I32, [P, P, P, P, P, I64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]),
+        "edk_lindblad_dim": (I32, [P, C.POINTER(C.c_int)]),
+        "edk_lindblad_destroy": (I32, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -293,6 +293,56 @@ int edk_krylov_accepted_intervals(edk_krylov *c, double *out, int64_t cap,
                                   int64_t *n);
 int edk_krylov_destroy(edk_krylov *c);
 
+/* ---- Lindblad ------------------------------------------------------------- */
+/* LiouvillianMap(H, jumps) (LindbladEvolution.jl:62-92): the many-body
+ * Lindbladian on vec(rho), rho a dense d x d density matrix.  H and the nj
+ * jump operators: device complex128 column-major d x d matrices, the jumps
+ * stored one after another (nj d^2 elements); both are copied. */
+typedef struct edk_liouvillian edk_liouvillian;
+int edk_liouvillian_create(edk_ctx *ctx, int64_t d, const void *H, int nj, const void *jumps,
+                           edk_liouvillian **out);
+/* A * rho (_liouvillian_apply!, LindbladEvolution.jl:114-151): y = L(x),
+ * x and y distinct device d x d buffers */
+int edk_liouvillian_apply(edk_liouvillian *A, const void *x, void *y);
+int edk_liouvillian_destroy(edk_liouvillian *A);
+
+/* LindbladArnoldiCache keywords (LindbladEvolution.jl:154-174, 216-228) */
+typedef struct {
+  double tol;               /* 1e-10 */
+  int32_t m_init;           /* 20 */
+  int32_t m_max;            /* 50 */
+  int32_t extend_step;      /* 10 */
+  int32_t nsample;          /* 17 */
+  int32_t bisect_iters;     /* 12 */
+  int32_t extend_basis;     /* 1 */
+  int32_t normalize_trace;  /* 0 */
+  int32_t record_min_eig;   /* 0 */
+} edk_lindblad_params;
+/* LindbladArnoldiDiagnostics counters (LindbladEvolution.jl:17-37); the
+ * per-output vectors come from edk_lindblad_records */
+typedef struct {
+  int64_t basis_builds, basis_extensions, restarts, liouvillian_applies,
+      total_times_served, max_dim_used, n_accepted_intervals;
+  double max_trace_drift, max_hermiticity_drift, min_min_hermitian_eig;
+} edk_lindblad_diag;
+typedef struct edk_lindblad edk_lindblad;
+void edk_lindblad_default_params(edk_lindblad_params *p);
+/* LindbladArnoldiCache(A, dm; kwargs...) (LindbladEvolution.jl:216-273);
+ * rho0: device d x d complex128, column-major.  The cache keeps A. */
+int edk_lindblad_create(edk_liouvillian *A, const void *rho0, const edk_lindblad_params *params,
+                        edk_lindblad **out);
+/* lindblad_timeevolve!(cache, t) (LindbladEvolution.jl:598-611); out: device d x d */
+int edk_lindblad_evolve(edk_lindblad *c, double t, void *out);
+int edk_lindblad_diagnostics(edk_lindblad *c, edk_lindblad_diag *d);
+/* accepted_intervals, trace_drifts, hermiticity_drifts and min_hermitian_eigs
+ * (up to cap entries each; any pointer may be null); their lengths in *n_* */
+int edk_lindblad_records(edk_lindblad *c, double *accepted, double *trace_drifts, double *herm_drifts,
+                         double *min_eigs, int64_t cap, int64_t *n_accepted, int64_t *n_outputs,
+                         int64_t *n_eigs);
+/* current Arnoldi dimension (cache.m) */
+int edk_lindblad_dim(edk_lindblad *c, int *m);
+int edk_lindblad_destroy(edk_lindblad *c);
+
 /* ---- serving --------------------------------------------------------------- */
 /* No counterpart in the reference.  A server runs Lanczos ground-state and
  * Krylov time-evolution jobs (any sector and operator on its device) on
