@@ -48,17 +48,22 @@ class Lindblad:
     operators, as dense matrices."""
 
     def __init__(self, H, L):
-        self.H = np.asarray(H)
-        self.L = [np.asarray(x) for x in L]
+        self.H = _dense(H)
+        self.L = [_dense(x) for x in L]
 
 
 def lindblad(H, L):
     return Lindblad(H, L)
 
 
+def _dense(a):
+    """Dense form of a matrix argument (scipy sparse matrices are densified)."""
+    return a.toarray() if hasattr(a, "toarray") else np.asarray(a)
+
+
 def _colmajor(a):
     """Device layout of a matrix: column-major, complex128."""
-    return np.ascontiguousarray(np.asarray(a, np.complex128).T)
+    return np.ascontiguousarray(np.asarray(_dense(a), np.complex128).T)
 
 
 def _rho_of(dm):
@@ -71,8 +76,8 @@ class LiouvillianMap:
     def __init__(self, H, jumps=None, ctx=None):
         if isinstance(H, Lindblad):
             H, jumps = H.H, H.L
-        H = np.asarray(H)
-        jumps = [np.asarray(L) for L in (jumps or [])]
+        H = _dense(H)
+        jumps = [_dense(L) for L in (jumps or [])]
         if H.ndim != 2 or H.shape[0] != H.shape[1]:
             raise ValueError("LiouvillianMap: H must be square")
         d = H.shape[0]
