@@ -6,7 +6,7 @@ TranslationalBasis, TranslationParityBasis, ParityBasis, FlipBasis,
 ParityFlipBasis, TranslationFlipBasis, basis), spin, operator,
 trans_inv_operator, mul_ (mul!), H @ x / H * x, mul, timeevolve,
 KrylovEvolutionCache, timeevolve_ (timeevolve!), productstate, schmidt,
-ent_spec, ent_S, lindblad, densitymatrix, LiouvillianMap,
+ent_spec, ent_S, DoubleBasis, symmetrizer, basis_embedding, lindblad, densitymatrix, LiouvillianMap,
 LindbladArnoldiCache, lindblad_timeevolve, lindblad_timeevolve_
 (lindblad_timeevolve!), plus the new lanczos_ground.
 """
@@ -17,6 +17,7 @@ from .basis import (AbelianBasis, AbstractBasis, FlipBasis, ParityBasis,  # noqa
                     ParityFlipBasis, ProjectedBasis, TensorBasis, TranslationalBasis,
                     TranslationFlipBasis, TranslationParityBasis, basis, productstate)
 from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401
+from .linearmap import DoubleBasis, basis_embedding, symmetrizer  # noqa: F401
 from .lindblad import (DensityMatrix, Lindblad, LindbladArnoldiCache, LiouvillianMap,  # noqa: F401
                        densitymatrix, lindblad, lindblad_timeevolve, lindblad_timeevolve_)
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401

@@ -38,6 +38,7 @@ SYMBOLS = [
     "edk_schmidt", "edk_ent_spec",
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
     "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
+    "edk_double_operator_create", "edk_double_matvec", "edk_double_operator_destroy", "edk_double_map",
     "edk_liouvillian_create", "edk_liouvillian_apply", "edk_liouvillian_destroy",
     "edk_lindblad_default_params", "edk_lindblad_create", "edk_lindblad_evolve",
     "edk_lindblad_diagnostics", "edk_lindblad_records", "edk_lindblad_dim", "edk_lindblad_destroy",
@@ -186,6 +187,10 @@ def lib():
         "edk_krylov_diagnostics": (I32, [P, C.POINTER(KrylovDiag)]),
         "edk_krylov_accepted_intervals": (I32, [P, P, I64, C.POINTER(C.c_int64)]),
         "edk_krylov_destroy": (I32, [P]),
+        "edk_double_operator_create": (I32, [P, P, P, I64, P, P, P, P, P, P, P, PP]),
+        "edk_double_matvec": (I32, [P, I32, P, P, I64, I32]),
+        "edk_double_operator_destroy": (I32, [P]),
+        "edk_double_map": (I32, [P, P, I32, P, P, I64]),
         "edk_liouvillian_create": (I32, [P, I64, P, I32, P, PP]),
         "edk_liouvillian_apply": (I32, [P, P, P]),
         "edk_liouvillian_destroy": (I32, [P]),

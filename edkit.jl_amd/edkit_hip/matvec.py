@@ -12,6 +12,7 @@ import numpy as np
 
 from ._lib import check, lib
 from .device import DeviceArray, dev_ptr, dtype_code
+from .linearmap import double_apply, double_array, is_double
 
 
 class DeviceOperator:
@@ -92,6 +93,8 @@ def _nb(shape, D):
 
 def mul_(target, H, v):
     """mul!(target, H, v): target += H v (Operator.jl:309-328)."""
+    if is_double(H):
+        return double_apply(H, v, target=target, accumulate=True)
     op = device_operator(H)
     D = H.B.D
     if isinstance(v, np.ndarray):
@@ -112,6 +115,8 @@ def mul_(target, H, v):
 
 def apply(H, v):
     """H * v with a zero-initialised target of the promoted type (Operator.jl:372-394)."""
+    if is_double(H):
+        return double_apply(H, v)
     op = device_operator(H)
     D = H.B.D
     if isinstance(v, np.ndarray):
@@ -128,6 +133,8 @@ def apply(H, v):
 
 def matvec_into(y, H, x, accumulate=False):
     """Device y = (accumulate ? y : 0) + H x without allocation."""
+    if is_double(H):
+        return double_apply(H, x, target=y, accumulate=accumulate)
     op = device_operator(H)
     nb = _nb(x.shape, H.B.D) if hasattr(x, "shape") else 1
     check(lib().edk_matvec(op.h, dtype_code(x.dtype), dev_ptr(y), dev_ptr(x), nb, int(accumulate)))
@@ -135,12 +142,18 @@ def matvec_into(y, H, x, accumulate=False):
 
 
 def sparse(H):
-    """sparse(opt) (Operator.jl:210-216): the sector matrix, assembled on the device."""
+    """sparse(opt) (Operator.jl:210-216): the sector matrix, assembled on the device
+    (on a DoubleBasis: the rectangular matrix from its columns)."""
+    if is_double(H):
+        import scipy.sparse as sp
+        return sp.csr_matrix(double_array(H))
     return device_operator(H).csr()
 
 
 def Array(H):
     """Array(opt) (Operator.jl:188-194): the dense sector matrix."""
+    if is_double(H):
+        return double_array(H)
     return sparse(H).toarray()
 
 

@@ -67,6 +67,8 @@ class Operator:
 
     @property
     def shape(self):
+        if hasattr(self.B, "B1"):  # DoubleBasis: size(B1) x size(B2)
+            return self.B.shape
         D = self.B.D
         return (D, D)
 

@@ -293,6 +293,26 @@ int edk_krylov_accepted_intervals(edk_krylov *c, double *out, int64_t cap,
                                   int64_t *n);
 int edk_krylov_destroy(edk_krylov *c);
 
+/* ---- DoubleBasis ---------------------------------------------------------- */
+/* Operator(M, I, DoubleBasis(B1, B2)) (LinearMap.jl:88-126 with
+ * Operator.jl:24-78): the rectangular map from the source sector B2 to the
+ * target sector B1 (size(B1) x size(B2)).  Same term arrays as
+ * edk_operator_create; both bases must have the same L and base
+ * (check_doublebasis_compatible, LinearMap.jl:39-43) and device-built tables. */
+typedef struct edk_dop edk_dop;
+int edk_double_operator_create(edk_ctx *ctx, edk_basis *b1, edk_basis *b2, int64_t nterms,
+                               const int64_t *site_off, const int64_t *sites,
+                               const int64_t *colptr_off, const int64_t *colptr,
+                               const int64_t *nz_off, const int64_t *rowval,
+                               const double *nzval, edk_dop **out);
+/* mul!(y, H, x) / H * x on a DoubleBasis (Operator.jl:309-394): device
+ * vectors, y size(B1) x nb, x size(B2) x nb, columns row-interleaved */
+int edk_double_matvec(edk_dop *op, int dtype, void *y, const void *x, int64_t nb, int accumulate);
+int edk_double_operator_destroy(edk_dop *op);
+/* (B::DoubleBasis)(v) (LinearMap.jl:144-172): the basis-overlap map of
+ * device vectors v (size(B2) x nb) into out (size(B1) x nb) */
+int edk_double_map(edk_basis *b1, edk_basis *b2, int dtype, const void *v, void *out, int64_t nb);
+
 /* ---- Lindblad ------------------------------------------------------------- */
 /* LiouvillianMap(H, jumps) (LindbladEvolution.jl:62-92): the many-body
  * Lindbladian on vec(rho), rho a dense d x d density matrix.  H and the nj
