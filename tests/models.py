@@ -41,3 +41,29 @@ def rand_state(D, seed, dtype=np.complex128):
     v = rng.standard_normal(D) + 1j * rng.standard_normal(D)
     v /= np.linalg.norm(v)
     return v.astype(dtype)
+
+
+def torus_sites(Lx, Ly):
+    """Site (x, y) of an Lx x Ly torus is 1 + x + Lx y (x fastest)."""
+    return [(x, y) for y in range(Ly) for x in range(Lx)]
+
+
+def torus_translations(Lx, Ly):
+    """T_x and T_y as site permutations (content of site i moves to site
+    perm[i]), the form of the 2D example of basis(...) (AbelianBasis.jl:754-760)."""
+    sites = torus_sites(Lx, Ly)
+    tx = [(x + 1) % Lx + Lx * y + 1 for (x, y) in sites]
+    ty = [x + Lx * ((y + 1) % Ly) + 1 for (x, y) in sites]
+    return tx, ty
+
+
+def heisenberg_2d(B, Lx, Ly, j=1.0, delta=1.0):
+    """Nearest-neighbour XXZ model on an Lx x Ly torus (one bond per site and
+    direction; Lx, Ly >= 3 so no bond repeats)."""
+    h = j * spin((1.0, "xx"), (1.0, "yy"), (delta, "zz"))
+    bonds = []
+    for (x, y) in torus_sites(Lx, Ly):
+        s = 1 + x + Lx * y
+        bonds.append([s, 1 + (x + 1) % Lx + Lx * y])
+        bonds.append([s, 1 + x + Lx * ((y + 1) % Ly)])
+    return operator([h] * len(bonds), bonds, B)
