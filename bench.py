@@ -535,7 +535,8 @@ def c6_workload(args, ev):
     flops (complex multiply-adds of its GEMMs) against the FP64 matrix-core
     peak, over the apply's mean duration (HIP events on the map's stream).
     The CPU leg times the oracle's apply (numpy/BLAS, threaded) and charges it
-    per Liouvillian apply the device run made."""
+    per Liouvillian apply the device run made.  The "sparse" object runs the
+    same model through the sparse form (the jumps and H as CSR), HBM-bound."""
     import edkit_hip as edk
     from edkit_hip.lindblad import _colmajor
     sys.path.insert(0, ROOT)
@@ -591,6 +592,38 @@ def c6_workload(args, ev):
         "trace_last": float(np.real(np.trace(last))),
         "excited_last": float(np.real(np.trace(last @ sum(np.kron(np.kron(np.eye(2 ** i), np.diag([1.0, 0.0])),
                                                                    np.eye(2 ** (Ls - i - 1))) for i in range(Ls))))),
+    }
+    # the same model through the sparse form (CSR row/column passes over rho)
+    import scipy.sparse as sp
+    As = edk.LiouvillianMap(sp.csr_matrix(H), [sp.csr_matrix(Lk) for Lk in J])
+    nnz = sum(int(m.nnz) for m in [sp.csr_matrix(Lk) for Lk in J])
+    ys = edk.DeviceArray((d, d), np.complex128, ctx=As.ctx, zero=False)
+    for _ in range(3):
+        As.apply_device(x, ys)
+    ev.record(c0, stream)
+    for _ in range(n_ap):
+        As.apply_device(x, ys)
+    ev.record(c1, stream)
+    sp_ms = ev.elapsed_ms(c0, c1) / n_ap
+    # algorithmic bytes: every pass reads its dense operand once and writes
+    # (left: K1 X, L_k X) or updates (right: X K2, T L_k^H) the d x d output,
+    # plus the CSR entries (4 + 16 bytes) and row pointers it reads
+    Dm = sum((sp.csr_matrix(Lk).conj().T @ sp.csr_matrix(Lk) for Lk in J), sp.csr_matrix((d, d), dtype=complex))
+    k1nnz = int((sp.csr_matrix(H) * -1j - 0.5 * Dm).nnz)
+    d2 = 16.0 * d * d
+    sp_bytes = (2 * d2 + 3 * d2) * (1 + len(J)) + 20.0 * (2 * k1nnz + 2 * nnz) + 8.0 * (d + 1) * (2 + 2 * len(J))
+    ev.device_sync()
+    t0 = time.perf_counter()
+    cs = edk.LindbladArnoldiCache(As, r0)
+    outs_s = [cs.evolve_device(t) for t in ts]
+    ev.device_sync()
+    secs_s = time.perf_counter() - t0
+    dgs = cs.diagnostics
+    out["sparse"] = {
+        "apply_ms": sp_ms, "bytes_per_apply": sp_bytes, "GBs": sp_bytes / (sp_ms * 1e-3) / 1e9,
+        "frac_hbm": sp_bytes / (sp_ms * 1e-3) / 1e9 / 8000.0, "outputs_per_s": len(ts) / secs_s,
+        "liouvillian_applies": dgs["liouvillian_applies"], "restarts": dgs["restarts"],
+        "max_diff_vs_dense": float(max(np.abs(a.to_host() - b.to_host()).max() for a, b in zip(outs, outs_s))),
     }
     if not args.no_cpu:
         from oracle import lindblad as ol

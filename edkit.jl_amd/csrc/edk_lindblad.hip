@@ -31,6 +31,7 @@
 #include <complex>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -136,6 +137,53 @@ __global__ void k_lv_cscale(double2 *y, double2 a, int64_t n) {
   y[i] = make_double2(a.x * v.x - a.y * v.y, a.x * v.y + a.y * v.x);
 }
 
+// Sparse products with a dense column-major d x d X (element (r, c) at r + c d).
+// One lane per (r, c), r fastest; every lane sums its terms in a fixed order.
+//   left:  Y[r, c] (+)= sum_e v[e] X[ci[e], c] over row r of A   (Y = A X)
+//   right: Y[r, c] += sum_e X[r, ci[e]] f(v[e]) over row c of M  (Y += X M^T, f = conj: X M^H)
+// The right form reads row c of M uniformly across the block and X[r, k]
+// coalesced; the left form gathers within the contiguous column c.
+__global__ void __launch_bounds__(256) k_lv_spmm_left(double2 *__restrict__ Y, const int *__restrict__ rp,
+                                                      const int *__restrict__ ci, const double2 *__restrict__ v,
+                                                      const double2 *__restrict__ X, int64_t d, int accumulate) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= d) return;
+  const int64_t c = blockIdx.y;
+  const double2 *xc = X + c * d;
+  double ar = 0.0, ai = 0.0;
+  for (int e = rp[r]; e < rp[r + 1]; e++) {
+    const double2 a = v[e], x = xc[ci[e]];
+    ar += a.x * x.x - a.y * x.y;
+    ai += a.x * x.y + a.y * x.x;
+  }
+  double2 *yp = Y + r + c * d;
+  if (accumulate) {
+    const double2 y0 = *yp;
+    ar += y0.x;
+    ai += y0.y;
+  }
+  *yp = make_double2(ar, ai);
+}
+
+__global__ void __launch_bounds__(256) k_lv_spmm_right(double2 *__restrict__ Y, const int *__restrict__ rp,
+                                                       const int *__restrict__ ci, const double2 *__restrict__ v,
+                                                       const double2 *__restrict__ X, int64_t d, int conj) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= d) return;
+  const int64_t c = blockIdx.y;
+  const double sg = conj ? -1.0 : 1.0;
+  double ar = 0.0, ai = 0.0;
+  for (int e = rp[c]; e < rp[c + 1]; e++) {
+    const double2 m = v[e], x = X[r + (int64_t)ci[e] * d];
+    const double mi = sg * m.y;
+    ar += x.x * m.x - x.y * mi;
+    ai += x.x * mi + x.y * m.x;
+  }
+  double2 *yp = Y + r + c * d;
+  const double2 y0 = *yp;
+  *yp = make_double2(y0.x + ar, y0.y + ai);
+}
+
 // ---------------------------------------------------------------------------
 // LiouvillianMap
 // ---------------------------------------------------------------------------
@@ -143,9 +191,16 @@ struct edk_liouvillian {
   edk_ctx *ctx = nullptr;
   int64_t d = 0;
   int nj = 0;
+  // dense form
   double2 *Lb = nullptr;  // [K1 | L_1 | ... | L_nj]: d x (nj+1) d, column-major
   double2 *K2 = nullptr;  // i H - D/2
   double2 *Rb = nullptr;  // [X; X L_1^H; ...; X L_nj^H]: (nj+1) d x d
+  // sparse form: CSR of K1, of K2^T and of every jump (rows of all
+  // matrices back to back: matrix q has rows rp[q (d+1) ...])
+  bool sparse = false;
+  int *rp = nullptr, *ci = nullptr;
+  double2 *cv = nullptr;
+  double2 *T = nullptr;   // L_k X
   rocblas_handle h = nullptr;
 };
 
@@ -155,6 +210,10 @@ static void lv_free(edk_liouvillian *A) {
   edk_dfree(A->ctx, A->Lb);
   edk_dfree(A->ctx, A->K2);
   edk_dfree(A->ctx, A->Rb);
+  edk_dfree(A->ctx, A->rp);
+  edk_dfree(A->ctx, A->ci);
+  edk_dfree(A->ctx, A->cv);
+  edk_dfree(A->ctx, A->T);
   delete A;
 }
 
@@ -162,7 +221,26 @@ static const rocblas_double_complex *rbc(const void *p) { return (const rocblas_
 static rocblas_double_complex *rbm(void *p) { return (rocblas_double_complex *)p; }
 
 // y = L(x): x, y device d x d (column-major), distinct
+// sparse form: Y = K1 X + X K2 + sum_k (L_k X) L_k^H, 2 + 2 nj passes over d^2
+static int lv_apply_sparse(edk_liouvillian *A, const void *x, void *y) {
+  const int64_t d = A->d, d1 = d + 1;
+  hipStream_t s = A->ctx->stream;
+  const dim3 grid((unsigned)((d + 255) / 256), (unsigned)d);
+  const double2 *X = (const double2 *)x;
+  double2 *Y = (double2 *)y;
+  k_lv_spmm_left<<<grid, 256, 0, s>>>(Y, A->rp, A->ci, A->cv, X, d, 0);
+  k_lv_spmm_right<<<grid, 256, 0, s>>>(Y, A->rp + d1, A->ci, A->cv, X, d, 0);
+  for (int k = 0; k < A->nj; k++) {
+    const int *rk = A->rp + (2 + k) * d1;
+    k_lv_spmm_left<<<grid, 256, 0, s>>>(A->T, rk, A->ci, A->cv, X, d, 0);
+    k_lv_spmm_right<<<grid, 256, 0, s>>>(Y, rk, A->ci, A->cv, A->T, d, 1);
+  }
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+
 static int lv_apply(edk_liouvillian *A, const void *x, void *y) {
+  if (A->sparse) return lv_apply_sparse(A, x, y);
   const int64_t d = A->d, dd = d * d;
   const int nj = A->nj;
   const rocblas_int n = (rocblas_int)d, ldr = (rocblas_int)((nj + 1) * d);
@@ -231,6 +309,116 @@ extern "C" int edk_liouvillian_create(edk_ctx *ctx, int64_t d, const void *H, in
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
       rc = edk_fail(EDK_ERR_HIP, "LiouvillianMap set-up");
   }
+  if (rc) {
+    lv_free(A);
+    return rc;
+  }
+  *out = A;
+  return EDK_OK;
+}
+
+// LiouvillianMap(sparse(H), sparse.(jumps)) (LindbladEvolution.jl:78-92 with
+// issparse): host CSR arrays, 0-based, complex values interleaved.  H has
+// rows h_rp[0..d]; jump q has rows j_rp[q (d+1) ...] with offsets into
+// j_col / j_val.  D = sum_k L_k^H L_k, K1 = -iH - D/2 and K2^T are formed
+// here (sparse), then everything is copied to the device.
+extern "C" int edk_liouvillian_create_csr(edk_ctx *ctx, int64_t d, const int64_t *h_rp, const int32_t *h_col,
+                                          const double *h_val, int nj, const int64_t *j_rp, const int32_t *j_col,
+                                          const double *j_val, edk_liouvillian **out) {
+  if (!ctx || !h_rp || !out || (nj > 0 && !j_rp)) return edk_fail(EDK_ERR_ARG, "edk_liouvillian_create_csr: null argument");
+  if (d < 1) return edk_fail(EDK_ERR_ARG, "LiouvillianMap: H must be non-empty");
+  if (nj < 0) return edk_fail(EDK_ERR_ARG, "LiouvillianMap: negative number of jumps");
+  if (d > 46340) return edk_fail(EDK_ERR_UNSUPPORTED, "LiouvillianMap: dimension above 46340");
+  using Row = std::map<int, cplx>;
+  auto check_rows = [&](const int64_t *rp, const int32_t *col, const char *what) -> int {
+    if (rp[0] < 0) return edk_fail(EDK_ERR_ARG, std::string(what) + ": negative row pointer");
+    for (int64_t r = 0; r < d; r++) {
+      if (rp[r + 1] < rp[r]) return edk_fail(EDK_ERR_ARG, std::string(what) + ": decreasing row pointers");
+      for (int64_t e = rp[r]; e < rp[r + 1]; e++)
+        if (col[e] < 0 || col[e] >= d) return edk_fail(EDK_ERR_ARG, std::string(what) + ": column out of range");
+    }
+    return EDK_OK;
+  };
+  if (int rc = check_rows(h_rp, h_col, "H")) return rc;
+  for (int q = 0; q < nj; q++)
+    if (int rc = check_rows(j_rp + (int64_t)q * (d + 1), j_col, "jump")) return rc;
+  // D[i, j] = sum_k sum_r conj(L_k[r, i]) L_k[r, j]
+  std::vector<Row> D(d), K1(d), K2T(d);
+  for (int q = 0; q < nj; q++) {
+    const int64_t *rp = j_rp + (int64_t)q * (d + 1);
+    for (int64_t r = 0; r < d; r++)
+      for (int64_t a = rp[r]; a < rp[r + 1]; a++) {
+        const cplx la = std::conj(cplx(j_val[2 * a], j_val[2 * a + 1]));
+        for (int64_t b = rp[r]; b < rp[r + 1]; b++) D[j_col[a]][j_col[b]] += la * cplx(j_val[2 * b], j_val[2 * b + 1]);
+      }
+  }
+  for (int64_t r = 0; r < d; r++) {
+    for (int64_t e = h_rp[r]; e < h_rp[r + 1]; e++) {
+      const cplx h(h_val[2 * e], h_val[2 * e + 1]);
+      K1[r][h_col[e]] += cplx(0, -1) * h;
+      K2T[h_col[e]][r] += cplx(0, 1) * h;
+    }
+    for (auto &kv : D[r]) {
+      K1[r][kv.first] += -0.5 * kv.second;
+      K2T[kv.first][r] += -0.5 * kv.second;
+    }
+  }
+  // flatten: K1, K2^T, then the jumps as given
+  std::vector<int> rp;
+  std::vector<int> ci;
+  std::vector<double2> cv;
+  auto push_rows = [&](const std::vector<Row> &M) {
+    for (int64_t r = 0; r < d; r++) {
+      rp.push_back((int)ci.size());
+      for (auto &kv : M[r]) {
+        ci.push_back(kv.first);
+        cv.push_back(make_double2(kv.second.real(), kv.second.imag()));
+      }
+    }
+    rp.push_back((int)ci.size());
+  };
+  size_t total = 0;
+  for (int64_t r = 0; r < d; r++) total += K1[r].size() + K2T[r].size();
+  for (int q = 0; q < nj; q++) total += (size_t)(j_rp[(int64_t)q * (d + 1) + d] - j_rp[(int64_t)q * (d + 1)]);
+  if (total >= (size_t)INT32_MAX) return edk_fail(EDK_ERR_UNSUPPORTED, "LiouvillianMap: more than 2^31 stored entries");
+  push_rows(K1);
+  push_rows(K2T);
+  for (int q = 0; q < nj; q++) {
+    const int64_t *jr = j_rp + (int64_t)q * (d + 1);
+    for (int64_t r = 0; r < d; r++) {
+      rp.push_back((int)ci.size());
+      for (int64_t e = jr[r]; e < jr[r + 1]; e++) {
+        ci.push_back(j_col[e]);
+        cv.push_back(make_double2(j_val[2 * e], j_val[2 * e + 1]));
+      }
+    }
+    rp.push_back((int)ci.size());
+  }
+  EDK_HIP(hipSetDevice(ctx->device));
+  edk_liouvillian *A = new edk_liouvillian();
+  A->ctx = ctx;
+  A->d = d;
+  A->nj = nj;
+  A->sparse = true;
+  int rc = EDK_OK;
+  const size_t nnz = std::max<size_t>(ci.size(), 1);
+  if (edk_dalloc(ctx, (void **)&A->rp, rp.size() * sizeof(int)) != hipSuccess ||
+      edk_dalloc(ctx, (void **)&A->ci, nnz * sizeof(int)) != hipSuccess ||
+      edk_dalloc(ctx, (void **)&A->cv, nnz * sizeof(double2)) != hipSuccess ||
+      (nj > 0 && edk_dalloc(ctx, (void **)&A->T, (size_t)d * d * sizeof(double2)) != hipSuccess))
+    rc = edk_fail(EDK_ERR_OOM, "LiouvillianMap does not fit in device memory");
+  if (!rc && rocblas_create_handle(&A->h) != rocblas_status_success) {
+    A->h = nullptr;
+    rc = edk_fail(EDK_ERR_HIP, "rocblas_create_handle");
+  }
+  if (!rc && rocblas_set_stream(A->h, ctx->stream) != rocblas_status_success)
+    rc = edk_fail(EDK_ERR_HIP, "rocblas_set_stream");
+  hipStream_t st = ctx->stream;
+  if (!rc && (hipMemcpyAsync(A->rp, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+              (!ci.empty() && hipMemcpyAsync(A->ci, ci.data(), ci.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess) ||
+              (!cv.empty() && hipMemcpyAsync(A->cv, cv.data(), cv.size() * sizeof(double2), hipMemcpyHostToDevice, st) != hipSuccess) ||
+              hipStreamSynchronize(st) != hipSuccess))
+    rc = edk_fail(EDK_ERR_HIP, "copy of the sparse Liouvillian");
   if (rc) {
     lv_free(A);
     return rc;

@@ -39,7 +39,7 @@ SYMBOLS = [
     "edk_krylov_default_params", "edk_krylov_create", "edk_krylov_evolve",
     "edk_krylov_diagnostics", "edk_krylov_accepted_intervals", "edk_krylov_destroy",
     "edk_double_operator_create", "edk_double_matvec", "edk_double_operator_destroy", "edk_double_map",
-    "edk_liouvillian_create", "edk_liouvillian_apply", "edk_liouvillian_destroy",
+    "edk_liouvillian_create", "edk_liouvillian_create_csr", "edk_liouvillian_apply", "edk_liouvillian_destroy",
     "edk_lindblad_default_params", "edk_lindblad_create", "edk_lindblad_evolve",
     "edk_lindblad_diagnostics", "edk_lindblad_records", "edk_lindblad_dim", "edk_lindblad_destroy",
 ]
@@ -192,6 +192,7 @@ def lib():
         "edk_double_operator_destroy": (I32, [P]),
         "edk_double_map": (I32, [P, P, I32, P, P, I64]),
         "edk_liouvillian_create": (I32, [P, I64, P, I32, P, PP]),
+        "edk_liouvillian_create_csr": (I32, [P, I64, P, P, P, I32, P, P, P, PP]),
         "edk_liouvillian_apply": (I32, [P, P, P]),
         "edk_liouvillian_destroy": (I32, [P]),
         "edk_lindblad_default_params": (None, [C.POINTER(LindbladParams)]),

@@ -48,8 +48,8 @@ class Lindblad:
     operators, as dense matrices."""
 
     def __init__(self, H, L):
-        self.H = _dense(H)
-        self.L = [_dense(x) for x in L]
+        self.H = H if hasattr(H, "tocsr") else np.asarray(H)
+        self.L = [x if hasattr(x, "tocsr") else np.asarray(x) for x in L]
 
 
 def lindblad(H, L):
@@ -66,6 +66,16 @@ def _colmajor(a):
     return np.ascontiguousarray(np.asarray(_dense(a), np.complex128).T)
 
 
+def _csr(a):
+    """0-based CSR arrays (int64 row pointers, int32 columns, interleaved complex values)."""
+    import scipy.sparse as sp
+    m = sp.csr_matrix(a, dtype=np.complex128)
+    m.sort_indices()
+    vals = np.ascontiguousarray(m.data.astype(np.complex128)).view(np.float64)
+    return (np.ascontiguousarray(m.indptr.astype(np.int64)), np.ascontiguousarray(m.indices.astype(np.int32)),
+            np.ascontiguousarray(vals))
+
+
 def _rho_of(dm):
     return np.asarray(dm.rho if isinstance(dm, DensityMatrix) else dm)
 
@@ -73,24 +83,50 @@ def _rho_of(dm):
 class LiouvillianMap:
     """LiouvillianMap(H, jumps) or LiouvillianMap(lb) (LindbladEvolution.jl:62-92)."""
 
-    def __init__(self, H, jumps=None, ctx=None):
+    def __init__(self, H, jumps=None, ctx=None, sparse=None):
+        """sparse: None follows the input (scipy sparse H selects the sparse
+        form, as issparse(H) does in the reference); True/False forces it."""
         if isinstance(H, Lindblad):
             H, jumps = H.H, H.L
-        H = _dense(H)
-        jumps = [_dense(L) for L in (jumps or [])]
-        if H.ndim != 2 or H.shape[0] != H.shape[1]:
+        jumps = [] if jumps is None else list(jumps)
+        if sparse is None:
+            sparse = hasattr(H, "tocsr")
+        if not hasattr(H, "tocsr"):
+            H = np.asarray(H)
+        jumps = [L if hasattr(L, "tocsr") else np.asarray(L) for L in jumps]
+        shape = H.shape
+        if len(shape) != 2 or shape[0] != shape[1]:
             raise ValueError("LiouvillianMap: H must be square")
-        d = H.shape[0]
+        d = shape[0]
         for j, L in enumerate(jumps):
-            if L.shape != (d, d):
+            if tuple(L.shape) != (d, d):
                 raise ValueError(f"LiouvillianMap: jump {j + 1} has incompatible size")
         self.ctx = ctx or default_context()
         self.dim = d
         self.njumps = len(jumps)
-        dH = DeviceArray.from_host(_colmajor(H), ctx=self.ctx)
-        dJ = (DeviceArray.from_host(np.stack([_colmajor(L) for L in jumps]), ctx=self.ctx) if jumps else None)
+        self.sparse = bool(sparse)
         h = C.c_void_p()
-        check(lib().edk_liouvillian_create(self.ctx.h, d, dH.ptr, len(jumps), dJ.ptr if dJ else None, C.byref(h)))
+        if self.sparse:
+            hp, hc, hv = _csr(H)
+            keep = [hp, hc, hv]
+            if jumps:
+                parts = [_csr(L) for L in jumps]
+                offs = np.cumsum([0] + [p[1].size for p in parts])
+                jp = np.ascontiguousarray(np.concatenate([p[0] + o for p, o in zip(parts, offs[:-1])]))
+                jc = np.ascontiguousarray(np.concatenate([p[1] for p in parts]))
+                jv = np.ascontiguousarray(np.concatenate([p[2] for p in parts]))
+                keep += [jp, jc, jv]
+                jargs = (jp.ctypes.data, jc.ctypes.data, jv.ctypes.data)
+            else:
+                jargs = (None, None, None)
+            check(lib().edk_liouvillian_create_csr(self.ctx.h, d, hp.ctypes.data, hc.ctypes.data, hv.ctypes.data,
+                                                   len(jumps), *jargs, C.byref(h)))
+            del keep
+        else:
+            dH = DeviceArray.from_host(_colmajor(H), ctx=self.ctx)
+            dJ = (DeviceArray.from_host(np.stack([_colmajor(L) for L in jumps]), ctx=self.ctx) if jumps else None)
+            check(lib().edk_liouvillian_create(self.ctx.h, d, dH.ptr, len(jumps), dJ.ptr if dJ else None,
+                                               C.byref(h)))
         self.h = h
 
     @property

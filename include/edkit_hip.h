@@ -321,6 +321,13 @@ int edk_double_map(edk_basis *b1, edk_basis *b2, int dtype, const void *v, void 
 typedef struct edk_liouvillian edk_liouvillian;
 int edk_liouvillian_create(edk_ctx *ctx, int64_t d, const void *H, int nj, const void *jumps,
                            edk_liouvillian **out);
+/* LiouvillianMap(sparse(H), sparse.(jumps)): host CSR arrays, 0-based, complex
+ * values interleaved (re, im).  H: rows h_rp[0..d] into h_col/h_val; jump q:
+ * rows j_rp[q(d+1) .. q(d+1)+d] into j_col/j_val.  The products run as sparse
+ * row/column passes over the dense density matrix (O(nnz d) per apply). */
+int edk_liouvillian_create_csr(edk_ctx *ctx, int64_t d, const int64_t *h_rp, const int32_t *h_col,
+                               const double *h_val, int nj, const int64_t *j_rp, const int32_t *j_col,
+                               const double *j_val, edk_liouvillian **out);
 /* A * rho (_liouvillian_apply!, LindbladEvolution.jl:114-151): y = L(x),
  * x and y distinct device d x d buffers */
 int edk_liouvillian_apply(edk_liouvillian *A, const void *x, void *y);
