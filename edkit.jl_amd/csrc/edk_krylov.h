@@ -22,6 +22,8 @@ struct KrylovWork {
   size_t esize() const;
   int dot(const void *a, const void *b, double2 *out);  // sum conj(a) b (b == null: |a|^2)
   int norm(const void *a, double *out);
+  // w += alpha x, then out = y^H w over the updated w (y == null: out.x = |w|^2), one pass
+  int axpy_dot(void *w, const void *x, double2 alpha, const void *y, double2 *out);
   int axpy(void *y, const void *x, double2 alpha);
   int scale(void *y, const void *x, double s);
   int cgs_sweep(const void *V, int64_t ldv, int j, void *w);
@@ -55,8 +57,12 @@ struct PartialOrth {
   void reset(int j);
 };
 
+// brk < 0: the reference's breakdown rule beta <= sqrt(eps) max(1, |alpha|)
+// (TimeEvolution.jl:284-292).  brk >= 0 (ground state with a tolerance): a
+// beta below that bound is first orthogonalised against the whole basis and
+// counts as a breakdown only when the remainder is <= brk.
 int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta, int *m,
-                   int m_target, int64_t *matvecs, int ncols, PartialOrth *po = nullptr);
+                   int m_target, int64_t *matvecs, int ncols, PartialOrth *po = nullptr, double brk = -1.0);
 
 // the Lanczos ground state and the Krylov cache on a given context (its stream
 // and allocation mode), for callers that run several at once

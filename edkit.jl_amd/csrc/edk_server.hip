@@ -70,7 +70,10 @@ static size_t job_footprint(const edk_job &j) {
     cols = std::max<size_t>(cols, std::min<size_t>(j.m + 1, 64));
     // cached allocations round up by at most 1/8
     size_t f = D * es * (cols + 2) / 8 * 9 + parts;
-    if (j.refine || j.want_vector) f += 2 * D * 16;
+    // the Ritz vector and its product (refinement, or the true-residual check
+    // of a partially reorthogonalised ComplexF64 run with a tolerance) and a
+    // restart vector
+    if (j.refine || j.want_vector || (j.reorth == EDK_REORTH_PARTIAL && j.tol > 0)) f += 3 * D * 16;
     if (j.want_vector) f += D * es;
     return f;
   }

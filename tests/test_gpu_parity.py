@@ -342,3 +342,50 @@ def test_fast_and_generic_kernels_agree(gpu, monkeypatch):
             yg = model(B) @ x.astype(dt)
             tol = 1e-13 if dt == np.complex128 else 1e-6
             assert np.linalg.norm(yf - yg) <= tol * np.linalg.norm(yg), (kind, kw, dt)
+
+
+# ground-state jobs of the serving stream whose two lowest Ritz values nearly
+# coincide: (L, base, k, model, J2, start fraction)
+HARD_GROUND = [
+    (18, 2, 3, "j1j2", 0.38744123481230863, 0.29808910594826366),
+    (18, 2, 2, "j1j2", 0.2838251650945879, 0.47446615264950187),
+    (18, 2, 16, "j1j2", 0.35826390790807205, 0.38122420157129344),
+    (18, 2, 5, "j1j2", 0.5617494483597811, 0.19114534808061856),
+]
+
+
+@pytest.mark.parametrize("L,base,k,model,c,start", HARD_GROUND)
+@pytest.mark.parametrize("reorth", ["partial", "full"])
+def test_ground_stops_only_when_converged(gpu, L, base, k, model, c, start, reorth):
+    """A Lanczos run with a tolerance reaches a true (refined) residual
+    <= tol: its convergence test is confirmed by the QL solver, a beta below
+    sqrt(eps)|alpha| stops the run only after full orthogonalisation leaves
+    it within tol, and a partially reorthogonalised run whose Ritz vector
+    misses tol restarts from it with full reorthogonalisation.  (These
+    start vectors, overlapping the ground state by ~1e-5, used to stop at
+    true residuals of 1e-9 .. 1e-8.)  The energy is the sector's lowest
+    eigenvalue from dense eigh of the oracle's matrix."""
+    import edkit_hip as edk
+    from tests.models import blbq, j1j2
+    N = L // 2
+    B = edk.TranslationalBasis(L=L, N=N, k=k, base=base)
+    H = j1j2(B, c)
+    x0 = np.zeros(B.D, np.complex128)
+    x0[min(int(start * B.D), B.D - 1)] = 1
+    E, res, steps, _ = edk.lanczos_ground(H, x0, m=400, tol=1e-9, reorth=reorth)
+    assert res <= 1e-9 and steps < 400
+    assert abs(E - _dense_ground(L, base, k, c, H)) < 1e-9
+
+
+_DENSE_E0 = {}
+
+
+def _dense_ground(L, base, k, c, H):
+    """Lowest eigenvalue of the oracle's sector matrix (dense eigh), once per sector."""
+    key = (L, base, k, c)
+    if key not in _DENSE_E0:
+        ob = orc.Basis("translational", L, base=base, N=L // 2, k=k)
+        op = orc.Op(ob, H.flatten())
+        Hs = np.stack([op @ np.eye(ob.D, dtype=np.complex128)[:, j] for j in range(ob.D)], axis=1)
+        _DENSE_E0[key] = np.linalg.eigvalsh(0.5 * (Hs + Hs.conj().T))[0]
+    return _DENSE_E0[key]
