@@ -245,9 +245,12 @@ int edk_ent_spec(edk_basis *b, int dtype, const void *v, int nA, const int32_t *
 /* Lanczos ground state (no counterpart in the reference; built from the
  * _lanczos_build! recurrence TimeEvolution.jl:250-306 with full
  * reorthogonalisation).  x0: device vector (dtype); m: max steps; tol: stop
- * when |beta_j Q[j,1]| <= tol (tol <= 0 runs m steps); refine: 1 adds a final
- * FP64 Rayleigh quotient/residual with the ComplexF64 matvec on the Ritz
- * vector.  Outputs: E0, residual, steps; vec_out (device, dtype) optional. */
+ * when |beta_j Q[j,1]| <= tol, confirmed by the QL solver (tol <= 0 runs m
+ * steps; with tol > 0 a beta below sqrt(eps)|alpha| ends the run only when,
+ * after full orthogonalisation, it is <= tol); refine: 1 adds a final FP64
+ * Rayleigh quotient/residual with the ComplexF64 matvec on the Ritz vector.
+ * Outputs: E0, residual, steps (all restarts included); vec_out (device,
+ * dtype) optional. */
 int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m,
                        double tol, int refine, double *E0, double *resid,
                        int *steps, void *vec_out);
@@ -257,7 +260,10 @@ int edk_lanczos_ground(edk_op *op, int dtype, const void *x0, int m,
                                 orthogonality passes sqrt(eps) (then at that step
                                 and the next); the basis stays semi-orthogonal */
 /* edk_lanczos_ground with a choice of reorthogonalisation (no counterpart in
- * the reference).  reorth_steps (optional): steps that reorthogonalised. */
+ * the reference).  reorth_steps (optional): steps that reorthogonalised.
+ * EDK_REORTH_PARTIAL with tol > 0 (ComplexF64) checks the true residual of
+ * the Ritz vector when the estimate converged and, above tol, restarts from
+ * it with full reorthogonalisation for the remaining steps. */
 int edk_lanczos_ground_ex(edk_op *op, int dtype, const void *x0, int m,
                           double tol, int refine, int reorth, double *E0,
                           double *resid, int *steps, int *reorth_steps,

@@ -363,8 +363,8 @@ def test_ground_stops_only_when_converged(gpu, L, base, k, model, c, start, reor
     it within tol, and a partially reorthogonalised run whose Ritz vector
     misses tol restarts from it with full reorthogonalisation.  (These
     start vectors, overlapping the ground state by ~1e-5, used to stop at
-    true residuals of 1e-9 .. 1e-8.)  The energy is the sector's lowest
-    eigenvalue from dense eigh of the oracle's matrix."""
+    true residuals of 1e-9 .. 1e-8.)  The energy is the oracle's Lanczos
+    ground energy from a random start."""
     import edkit_hip as edk
     from tests.models import blbq, j1j2
     N = L // 2
@@ -374,18 +374,18 @@ def test_ground_stops_only_when_converged(gpu, L, base, k, model, c, start, reor
     x0[min(int(start * B.D), B.D - 1)] = 1
     E, res, steps, _ = edk.lanczos_ground(H, x0, m=400, tol=1e-9, reorth=reorth)
     assert res <= 1e-9 and steps < 400
-    assert abs(E - _dense_ground(L, base, k, c, H)) < 1e-9
+    assert abs(E - _oracle_ground(L, base, k, c, H)) < 1e-9
 
 
-_DENSE_E0 = {}
+_ORACLE_E0 = {}
 
 
-def _dense_ground(L, base, k, c, H):
-    """Lowest eigenvalue of the oracle's sector matrix (dense eigh), once per sector."""
+def _oracle_ground(L, base, k, c, H):
+    """The sector's ground energy from the oracle's Lanczos (300 steps, full
+    reorthogonalisation) from a random start, once per sector."""
     key = (L, base, k, c)
-    if key not in _DENSE_E0:
+    if key not in _ORACLE_E0:
         ob = orc.Basis("translational", L, base=base, N=L // 2, k=k)
         op = orc.Op(ob, H.flatten())
-        Hs = np.stack([op @ np.eye(ob.D, dtype=np.complex128)[:, j] for j in range(ob.D)], axis=1)
-        _DENSE_E0[key] = np.linalg.eigvalsh(0.5 * (Hs + Hs.conj().T))[0]
-    return _DENSE_E0[key]
+        _ORACLE_E0[key] = orc.lanczos_ground(op, rand_state(ob.D, 17), 300, 0.0, nthreads=8)[0]
+    return _ORACLE_E0[key]
