@@ -68,3 +68,33 @@ def test_torus_ground_state(gpu):
     e0 = np.linalg.eigvalsh(0.5 * (Hs + Hs.conj().T))[0]
     E0, res, steps, _ = edk.lanczos_ground(H, rand_state(B.D, 11), m=200, tol=1e-10)
     assert abs(E0 - e0) < 1e-9 and res < 1e-8
+
+
+def test_torus_double_basis_and_schmidt(gpu):
+    """Custom generators through the DoubleBasis orbit walk and the Schmidt
+    walker: the overlap map from a (kx, ky) torus sector into its charge
+    sector, S+ between two torus sectors at zero momentum (a covariant
+    operator), and the Schmidt matrix of a sector state over the first row."""
+    import edkit_hip as edk
+    from oracle import linearmap as olm
+    Lx, Ly = 4, 3
+    L = Lx * Ly
+    B1, ob1 = torus_pair(edk, Lx, Ly, 6, 1, 2, None)
+    Bp, obp = edk.ProjectedBasis(L=L, N=6), orc.Basis("projected", L, N=6)
+    v = rand_state(Bp.D, 21)
+    T = edk.DoubleBasis(B1, Bp)
+    ref = olm.double_map(ob1, obp, v)
+    assert np.abs(T(v) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    Bn, obn = torus_pair(edk, Lx, Ly, 5, 0, 0, None)
+    Bm, obm = torus_pair(edk, Lx, Ly, 6, 0, 0, None)
+    sp = np.array([[0, 1], [0, 0]], complex)
+    one = [[i] for i in range(1, L + 1)]
+    H = edk.operator([sp] * L, one, edk.DoubleBasis(Bm, Bn))
+    Hr = olm.double_operator(obm, obn, [sp] * L, one)
+    A = edk.Array(H)
+    assert A.shape == Hr.shape and np.abs(A - Hr).max() <= 1e-13 * max(1.0, np.abs(Hr).max())
+    w = rand_state(B1.D, 22)
+    Ainds = list(range(1, Lx + 1))
+    M = edk.schmidt(w, Ainds, B1)
+    Mo = ob1.schmidt(w, Ainds)
+    assert np.abs(M - Mo).max() <= 1e-14

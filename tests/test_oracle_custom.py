@@ -75,3 +75,32 @@ def test_torus_spectrum_union_over_momenta():
         np.testing.assert_allclose(Hs, Hs.conj().T, atol=1e-12)
         vals += list(np.linalg.eigvalsh(Hs))
     np.testing.assert_allclose(np.sort(vals), full, atol=1e-11)
+
+
+@pytest.mark.parametrize("kx,ky", [(0, 0), (1, 2), (3, 1)])
+def test_torus_schmidt_is_the_embedded_state(kx, ky):
+    """schmidt(v, A, b) for a custom-generator basis (4 x 3 torus, N = 6)
+    element by element against the full-space state conj(P) v, P the
+    reference's embedding coef / order(b), reshaped by the A and B digits."""
+    Lx, Ly, N = 4, 3, 6
+    L = Lx * Ly
+    tx, ty = torus_translations(Lx, Ly)
+    ob = orc.Basis("abelian", L, N=N, symmetries=[(tx, kx), (ty, ky)])
+    rng = np.random.default_rng(kx + 7 * ky)
+    v = rng.normal(size=ob.D) + 1j * rng.normal(size=ob.D)
+    v /= np.linalg.norm(v)
+    A = [1, 2, 3, 4, 6]
+    M = ob.schmidt(v, A)
+    n = 2 ** L
+    pos, coef = ob.index(np.arange(n, dtype=np.uint64))
+    P = np.zeros((n, ob.D), np.complex128)
+    nz = coef != 0
+    P[np.nonzero(nz)[0], pos[nz] - 1] = coef[nz] / ob.order
+    psi = P.conj() @ v
+    Bs = [q for q in range(1, L + 1) if q not in A]
+    Md = np.zeros_like(M)
+    for c in range(n):
+        d = dense.digits(c, L, 2)
+        Md[dense.content([d[q - 1] for q in A], 2), dense.content([d[q - 1] for q in Bs], 2)] = psi[c]
+    np.testing.assert_allclose(M, Md, atol=1e-14)
+    assert abs(np.linalg.norm(M) - 1) < 1e-12
