@@ -603,9 +603,9 @@ extern "C" void edk_lindblad_default_params(edk_lindblad_params *p) {
 
 static inline cplx &hred(edk_lindblad *c, int i, int j) { return c->Hred[i + (size_t)j * (c->p.m_max + 1)]; }
 
-// _arnoldi_build! (LindbladEvolution.jl:282-321): two Gram-Schmidt sweeps per
-// step (classical here, modified there; both accumulate the two sweeps'
-// coefficients into the Hessenberg column), breakdown below
+// _arnoldi_build! (LindbladEvolution.jl:282-321): Gram-Schmidt with the
+// coefficients accumulated into the Hessenberg column (classical sweeps with
+// a conditional second one here, two modified sweeps there), breakdown below
 // sqrt(eps) max(1, ||H[1:j, j]||)
 static int arnoldi_build(edk_lindblad *c, int m_target) {
   m_target = std::min(m_target, c->p.m_max);
@@ -618,14 +618,24 @@ static int arnoldi_build(edk_lindblad *c, int m_target) {
     if (int rc = lv_apply(c->A, vj, c->w)) return rc;
     c->diag.liouvillian_applies++;
     for (int i = 0; i <= c->p.m_max; i++) hred(c, i, j - 1) = 0;
+    // A classical Gram-Schmidt sweep, and a second one only when the first
+    // removed most of w (Daniel, Gragg, Kaufman and Stewart: ||w'|| <
+    // ||w|| / sqrt(2), with ||w||^2 = ||w'||^2 + ||h||^2); both sweeps'
+    // coefficients go into the Hessenberg column.  The reference runs two
+    // modified sweeps at every step; the entries agree to rounding.
+    double hnext = 0;
     for (int sweep = 0; sweep < 2; sweep++) {
       if (int rc = K.cgs_sweep(c->V, N, j, c->w)) return rc;
       EDK_HIP(hipMemcpyAsync(K.h_s, K.d_s, (size_t)j * sizeof(double2), hipMemcpyDeviceToHost, s));
       EDK_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i < j; i++) hred(c, i, j - 1) += cplx(K.h_s[i].x, K.h_s[i].y);
+      double hh = 0;
+      for (int i = 0; i < j; i++) {
+        hred(c, i, j - 1) += cplx(K.h_s[i].x, K.h_s[i].y);
+        hh += K.h_s[i].x * K.h_s[i].x + K.h_s[i].y * K.h_s[i].y;
+      }
+      if (int rc = K.norm(c->w, &hnext)) return rc;
+      if (hnext * hnext >= 0.5 * (hnext * hnext + hh)) break;
     }
-    double hnext;
-    if (int rc = K.norm(c->w, &hnext)) return rc;
     hred(c, j, j - 1) = hnext;
     c->m = j;
     double cn = 0;
