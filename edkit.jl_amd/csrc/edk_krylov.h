@@ -6,6 +6,8 @@
 #define EDK_KRYLOV_MAXM 1024
 // slot of d_s / h_s that holds a norm while d_s[0..j) holds coefficients
 #define NORM_SLOT (EDK_KRYLOV_MAXM + 1)
+// host slot (h_s only) receiving a Lanczos coefficient left on the device
+#define ALPHA_SLOT (EDK_KRYLOV_MAXM + 2)
 
 // Deterministic FP64 vector kernels over length-n vectors of one dtype.
 struct KrylovWork {
@@ -24,6 +26,10 @@ struct KrylovWork {
   int norm(const void *a, double *out);
   // w += alpha x, then out = y^H w over the updated w (y == null: out.x = |w|^2), one pass
   int axpy_dot(void *w, const void *x, double2 alpha, const void *y, double2 *out);
+  // conj(y) . (w += alpha x) (x == null: conj(y) . w) left in d_s[0], copied to h_s[ALPHA_SLOT]
+  int coef_async(void *w, const void *x, double2 alpha, const void *y);
+  // w -= Re(d_s[0]) x; with norm2_out, |w|^2 in the same pass (synchronised)
+  int sub_coef(void *w, const void *x, double *norm2_out);
   int axpy(void *y, const void *x, double2 alpha);
   int scale(void *y, const void *x, double s);
   int cgs_sweep(const void *V, int64_t ldv, int j, void *w);

@@ -100,8 +100,10 @@ __device__ __forceinline__ double2 rnd(double2 v) {
 // b's share of conj(y) . w (|w|^2 when y == null) over the stored w.  Same
 // rows per block and the same reduction tree as k_dot_part.
 template <typename VT>
-__global__ void __launch_bounds__(RED_TPB) k_axpy_dot_part(VT *w, const VT *x, double2 alpha, const VT *y, int64_t n,
-                                                           double2 *part) {
+__global__ void __launch_bounds__(RED_TPB) k_axpy_dot_part(VT *w, const VT *x, double2 alpha, const double2 *dalpha,
+                                                           double sign, const VT *y, int64_t n, double2 *part) {
+  // dalpha != null: alpha = sign * Re(dalpha[0]), a coefficient left on the device
+  if (dalpha) alpha = make_double2(sign * dalpha[0].x, 0.0);
   double2 acc = make_double2(0, 0);
   for (int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RED_TPB) {
     double2 a = ld(w, i), b = ld(x, i);
@@ -119,6 +121,18 @@ __global__ void __launch_bounds__(RED_TPB) k_axpy_dot_part(VT *w, const VT *x, d
   }
   double2 r = block_sum2(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// y += sign * Re(dalpha[0]) * x
+template <typename VT>
+__global__ void k_axpy_re_dev(VT *y, const VT *x, const double2 *dalpha, double sign, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = sign * dalpha[0].x;
+  double2 u = ld(y, i), b = ld(x, i);
+  u.x += a * b.x - 0.0 * b.y;
+  u.y += a * b.y + 0.0 * b.x;
+  st(y, i, u);
 }
 
 // y += alpha * x
@@ -234,6 +248,7 @@ int KrylovWork::init(edk_ctx *c, int64_t n_, int dtype_, int maxm, int slot) {
   if (edk_dalloc(ctx, (void **)&d_part, pb) != hipSuccess) return edk_fail(EDK_ERR_OOM, "krylov partials");
   // coefficients in d_s[0..maxm+1], a norm in d_s[NORM_SLOT]
   static_assert(NORM_SLOT < EDK_PINNED_SLOT, "norm slot must fit in the pinned buffer");
+  static_assert(ALPHA_SLOT < EDK_PINNED_SLOT, "coefficient slot must fit in the pinned buffer");
   if (edk_dalloc(ctx, (void **)&d_s, (NORM_SLOT + 1) * sizeof(double2)) != hipSuccess)
     return edk_fail(EDK_ERR_OOM, "krylov coefficients");
   h_s = (double2 *)ctx->pinned + (size_t)slot * EDK_PINNED_SLOT;
@@ -276,12 +291,49 @@ int KrylovWork::dot(const void *a, const void *b, double2 *out) {
 
 int KrylovWork::axpy_dot(void *w, const void *x, double2 alpha, const void *y, double2 *out) {
   hipStream_t s = ctx->stream;
-  DISPATCH(VT, k_axpy_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((VT *)w, (const VT *)x, alpha, (const VT *)y, n, d_part));
+  DISPATCH(VT, k_axpy_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((VT *)w, (const VT *)x, alpha, nullptr, 0.0,
+                                                            (const VT *)y, n, d_part));
   k_reduce_part<<<1, RED_TPB, 0, s>>>(d_part, nblk, d_s);
   EDK_HIP(hipGetLastError());
   EDK_HIP(hipMemcpyAsync(h_s, d_s, sizeof(double2), hipMemcpyDeviceToHost, s));
   EDK_HIP(hipStreamSynchronize(s));
   *out = h_s[0];
+  return EDK_OK;
+}
+
+// The Lanczos coefficient stays on the device: conj(y) . (w + alpha x) (or
+// conj(y) . w when x == null) reduced into d_s[0], and copied to
+// h_s[ALPHA_SLOT] in stream order (read it after the next synchronisation).
+int KrylovWork::coef_async(void *w, const void *x, double2 alpha, const void *y) {
+  hipStream_t s = ctx->stream;
+  if (x) {
+    DISPATCH(VT, k_axpy_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((VT *)w, (const VT *)x, alpha, nullptr, 0.0,
+                                                              (const VT *)y, n, d_part));
+  } else {
+    DISPATCH(VT, k_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)y, (const VT *)w, n, d_part));
+  }
+  k_reduce_part<<<1, RED_TPB, 0, s>>>(d_part, nblk, d_s);
+  EDK_HIP(hipGetLastError());
+  EDK_HIP(hipMemcpyAsync(h_s + ALPHA_SLOT, d_s, sizeof(double2), hipMemcpyDeviceToHost, s));
+  return EDK_OK;
+}
+
+// w -= Re(d_s[0]) x, then (norm_out != null) |w|^2, synchronised
+int KrylovWork::sub_coef(void *w, const void *x, double *norm2_out) {
+  hipStream_t s = ctx->stream;
+  if (norm2_out) {
+    DISPATCH(VT, k_axpy_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((VT *)w, (const VT *)x, make_double2(0, 0), d_s, -1.0,
+                                                              (const VT *)nullptr, n, d_part));
+    k_reduce_part<<<1, RED_TPB, 0, s>>>(d_part, nblk, d_s + NORM_SLOT);
+    EDK_HIP(hipGetLastError());
+    EDK_HIP(hipMemcpyAsync(h_s + NORM_SLOT, d_s + NORM_SLOT, sizeof(double2), hipMemcpyDeviceToHost, s));
+    EDK_HIP(hipStreamSynchronize(s));
+    *norm2_out = h_s[NORM_SLOT].x;
+  } else {
+    unsigned g = (unsigned)((n + 255) / 256);
+    DISPATCH(VT, k_axpy_re_dev<VT><<<g, 256, 0, s>>>((VT *)w, (const VT *)x, d_s, -1.0, n));
+    EDK_HIP(hipGetLastError());
+  }
   return EDK_OK;
 }
 
@@ -573,25 +625,24 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
     char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
     if (int rc = edk_matvec_launch(op, K.dtype, w, vj, 1, 0, K.ctx->stream)) return -rc;
     (*matvecs)++;
-    // w -= beta_{j-1} v_{j-1} and alpha_j = v_j^H w in one pass
-    double2 d;
-    if (j > 1) {
-      char *vjm1 = (char *)V + (size_t)(j - 2) * ldv * es;
-      if (int rc = K.axpy_dot(w, vjm1, make_double2(-beta[j - 2], 0.0), vj, &d)) return -rc;
-    } else {
-      if (int rc = K.dot(vj, w, &d)) return -rc;
-    }
-    double aj = d.x;
-    alpha[j - 1] = aj;
+    // w -= beta_{j-1} v_{j-1} and alpha_j = v_j^H w in one pass; alpha_j
+    // stays on the device for the next update (one host synchronisation per
+    // step, at the norm), its host copy arriving in stream order
+    char *vjm1 = j > 1 ? (char *)V + (size_t)(j - 2) * ldv * es : nullptr;
+    if (int rc = K.coef_async(w, vjm1, make_double2(j > 1 ? -beta[j - 2] : 0.0, 0.0), vj)) return -rc;
     double bj;
     if (!po) {
-      if (int rc = K.axpy(w, vj, make_double2(-aj, 0.0))) return -rc;
+      if (int rc = K.sub_coef(w, vj, nullptr)) return -rc;
       if (int rc = K.reorth_norm(V, ldv, j, w, &bj, nullptr)) return -rc;
     } else {
       // w -= alpha_j v_j and ||w|| in one pass
-      double2 nn;
-      if (int rc = K.axpy_dot(w, vj, make_double2(-aj, 0.0), nullptr, &nn)) return -rc;
-      bj = std::sqrt(nn.x);
+      double n2;
+      if (int rc = K.sub_coef(w, vj, &n2)) return -rc;
+      bj = std::sqrt(n2);
+    }
+    const double aj = K.h_s[ALPHA_SLOT].x;
+    alpha[j - 1] = aj;
+    if (po) {
       if (po->update(j, alpha, beta, bj)) {
         double hm = 0;
         if (int rc = K.reorth_norm(V, ldv, j, w, &bj, nullptr, &hm)) return -rc;
