@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "edk_internal.h"
 
@@ -496,11 +497,26 @@ static int add_generator(edk_basis *b, int order, int k, const int32_t *perm, co
   }
   std::vector<uint64_t> lm;
   std::vector<int> ls;
+  // the fields' bit displacements (site i sits at bit (L - i) w; its content
+  // moves to site perm[i]): few distinct ones (a torus translation along the
+  // fastest axis has two) give a shift-and-mask form
+  std::map<int, uint64_t> disp;
+  const uint64_t fmask = (1ull << w) - 1;
+  for (int i = 1; i <= L; i++) {
+    const int src = (L - i) * w, dst = (L - perm[i - 1]) * w;
+    disp[dst - src] |= fmask << src;
+  }
   if (ident) type = GEN_IDENT;
   else if (rev) type = GEN_REV;
   else if (rot > 0) {
     type = GEN_ROTL;  // content of site i moves to site i-a: a left rotation
     param = rot * w;
+  } else if ((int)disp.size() <= EDK_MAXSHIFTS) {
+    type = GEN_SHIFTS;
+    for (auto &kv : disp) {
+      lm.push_back(kv.second);
+      ls.push_back(kv.first);
+    }
   } else {
     benes_fields(perm, L, w, lm, ls);
   }

@@ -57,7 +57,10 @@ void edk_dfree(edk_ctx *c, void *p);
 size_t edk_cache_bytes(int device);
 
 // Generator kinds of the Abelian canonicaliser.
-enum { GEN_ROTL = 0, GEN_REV = 1, GEN_IDENT = 2, GEN_BENES = 3 };
+// GEN_SHIFTS: a site permutation whose fields move by at most EDK_MAXSHIFTS
+// distinct bit displacements, applied as OR_g shift(s & mask_g, d_g)
+enum { GEN_ROTL = 0, GEN_REV = 1, GEN_IDENT = 2, GEN_BENES = 3, GEN_SHIFTS = 4 };
+#define EDK_MAXSHIFTS 4
 
 // Uniform description of a basis' canonicaliser, passed by value to kernels.
 // Tables it points to are small device arrays owned by the basis.
@@ -169,6 +172,16 @@ __device__ __forceinline__ uint64_t edk_apply_gen(const CanonDev &C, int g, uint
       uint64_t dd = ((s >> sh) ^ s) & m;
       s = s ^ dd ^ (dd << sh);
     }
+  } else if (type == GEN_SHIFTS) {
+    // fields grouped by bit displacement (positive: towards the MSB)
+    int off = d[4], n = d[5];
+    uint64_t r = 0;
+    for (int q = 0; q < n; q++) {
+      const uint64_t m = s & C.gdata[off + q];
+      const int sh = C.gshift[off + q];
+      r |= sh >= 0 ? (m << sh) : (m >> -sh);
+    }
+    s = r;
   }
   // spin inversion: gdata[ngen-indexed inversion slot]
   // spin inversion d -> base-1-d on the flagged sites (apply_inv!,
@@ -266,7 +279,101 @@ __device__ __forceinline__ CanonOut edk_canon_tp(const CanonDev &C, uint64_t s) 
 // Generator descriptor (EDK_GDESC ints each): type, order, rotation bits,
 // inversion-mask slot, Benes layer offset, layer count, phase-table offset.
 #define EDK_GDESC 8
+
+// A generator held in registers for the walk (every form but Benes).
+struct GenReg {
+  int type, order, rot, n;
+  uint64_t inv;
+  uint64_t m[EDK_MAXSHIFTS];
+  int sh[EDK_MAXSHIFTS];
+};
+
+__device__ __forceinline__ GenReg edk_load_gen(const CanonDev &C, int g) {
+  const int *d = C.gdesc + EDK_GDESC * g;
+  GenReg r;
+  r.type = d[0];
+  r.order = d[1];
+  r.rot = d[2];
+  r.inv = C.gdata[d[3]];
+  r.n = r.type == GEN_SHIFTS ? d[5] : 0;
+#pragma unroll
+  for (int q = 0; q < EDK_MAXSHIFTS; q++) {
+    r.m[q] = q < r.n ? C.gdata[d[4] + q] : 0ull;
+    r.sh[q] = q < r.n ? C.gshift[d[4] + q] : 0;
+  }
+  return r;
+}
+
+// edk_apply_gen for a register-resident generator
+__device__ __forceinline__ uint64_t edk_apply_genreg(const CanonDev &C, const GenReg &r, uint64_t s) {
+  if (r.type == GEN_ROTL) {
+    const int tot = C.L * C.w;
+    if (r.rot) s = ((s << r.rot) | (s >> (tot - r.rot))) & C.mask;
+  } else if (r.type == GEN_REV) {
+    s = edk_revfields(s, C.L, C.w);
+  } else if (r.type == GEN_SHIFTS) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int q = 0; q < EDK_MAXSHIFTS; q++) {
+      if (q < r.n) {
+        const uint64_t m = s & r.m[q];
+        o |= r.sh[q] >= 0 ? (m << r.sh[q]) : (m >> -r.sh[q]);
+      }
+    }
+    s = o;
+  }
+  if (r.inv) {
+    if (C.base != 3) s ^= r.inv;
+    else s ^= r.inv & ~((s & C.lowfields) << 1);
+  }
+  return s;
+}
+
+// The walk of edk_canon_abelian with up to three generators, none of them a
+// Benes network, held in registers: nested loops over the exponents
+// (generator 0 fastest, the odometer order), so the loop control is uniform
+// and each element costs one generator application; the first strict
+// minimum is kept as its element number e = a0 + n0 (a1 + n1 a2).
+__device__ __forceinline__ CanonOut edk_canon_abelian3(const CanonDev &C, uint64_t s) {
+  const GenReg g0 = edk_load_gen(C, 0);
+  const GenReg g1 = C.ngen > 1 ? edk_load_gen(C, 1) : g0;
+  const GenReg g2 = C.ngen > 2 ? edk_load_gen(C, 2) : g0;
+  const int n0 = g0.order, n1 = C.ngen > 1 ? g1.order : 1, n2 = C.ngen > 2 ? g2.order : 1;
+  uint64_t best = s;
+  int be = 0, nstab = 0, e = 0;
+  uint64_t c2 = s;
+  for (int a2 = 0; a2 < n2; a2++) {
+    uint64_t c1 = c2;
+    for (int a1 = 0; a1 < n1; a1++) {
+      uint64_t c0 = c1;
+      for (int a0 = 0; a0 < n0; a0++, e++) {
+        nstab += c0 == s;
+        if (c0 < best) {
+          best = c0;
+          be = e;
+        }
+        if (a0 + 1 < n0) c0 = edk_apply_genreg(C, g0, c0);
+      }
+      if (a1 + 1 < n1) c1 = edk_apply_genreg(C, g1, c1);
+    }
+    if (a2 + 1 < n2) c2 = edk_apply_genreg(C, g2, c2);
+  }
+  const int m0 = be % n0 + 1, m1 = (be / n0) % n1 + 1, m2 = be / (n0 * n1) + 1;
+  CanonOut o;
+  o.rep = best;
+  o.ok = 1;
+  o.R = C.Rtab[nstab - 1];
+  double2 acc = C.ph[C.gdesc[6] + m0 - 1];
+  if (C.ngen > 1) acc = cmul(acc, C.ph[C.gdesc[EDK_GDESC + 6] + m1 - 1]);
+  if (C.ngen > 2) acc = cmul(acc, C.ph[C.gdesc[2 * EDK_GDESC + 6] + m2 - 1]);
+  o.phase = acc;
+  return o;
+}
+
 __device__ __forceinline__ CanonOut edk_canon_abelian(const CanonDev &C, uint64_t s) {
+  if (C.ngen >= 1 && C.ngen <= 3 && C.gdesc[0] != GEN_BENES && (C.ngen < 2 || C.gdesc[EDK_GDESC] != GEN_BENES) &&
+      (C.ngen < 3 || C.gdesc[2 * EDK_GDESC] != GEN_BENES))
+    return edk_canon_abelian3(C, s);
   int os[EDK_MAXGEN], ms[EDK_MAXGEN];
   for (int g = 0; g < C.ngen; g++) os[g] = ms[g] = 1;
   uint64_t best = s, cur = s;

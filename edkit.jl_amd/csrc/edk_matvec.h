@@ -12,6 +12,7 @@ struct TermsDev {
   const int *rowptr;       // CSR row pointers into col/val
   const int *col;          // local column c of M[r, c]
   const double2 *val;      // M[r, c]
+  int nmo, nrp, nnz;       // lengths of mrow_off, rowptr, col / val
 };
 
 struct MatvecArgs {

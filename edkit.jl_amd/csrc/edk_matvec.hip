@@ -26,12 +26,32 @@
 // columns [col, col + NV) of a row-interleaved block share each target's
 // canonical form and lookup.
 // ---------------------------------------------------------------------------
-template <typename VT, typename ST, int NV>
+// STAGED: the local-term CSR tables (every distinct matrix) are copied into
+// LDS first; the host picks it when they fit GEN_LDS_* entries.
+#define GEN_LDS_MO 64
+#define GEN_LDS_RP 512
+#define GEN_LDS_NZ 512
+template <typename VT, typename ST, int NV, bool STAGED>
 __global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *__restrict__ x, VT *__restrict__ y,
                                                         int64_t nb, int64_t col, int accumulate) {
+  __shared__ int s_mo[STAGED ? GEN_LDS_MO : 1], s_rp[STAGED ? GEN_LDS_RP : 1], s_col[STAGED ? GEN_LDS_NZ : 1];
+  __shared__ double2 s_val[STAGED ? GEN_LDS_NZ : 1];
+  TermsDev T = A.T;
+  if constexpr (STAGED) {
+    for (int k = threadIdx.x; k < T.nmo; k += blockDim.x) s_mo[k] = T.mrow_off[k];
+    for (int k = threadIdx.x; k < T.nrp; k += blockDim.x) s_rp[k] = T.rowptr[k];
+    for (int k = threadIdx.x; k < T.nnz; k += blockDim.x) {
+      s_col[k] = T.col[k];
+      s_val[k] = T.val[k];
+    }
+    __syncthreads();
+    T.mrow_off = s_mo;
+    T.rowptr = s_rp;
+    T.col = s_col;
+    T.val = s_val;
+  }
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.D) return;
-  const TermsDev &T = A.T;
   uint64_t fm = (1ull << A.w) - 1;
   uint64_t s = A.kind == EDK_TENSOR ? edk_content_to_packed_d((uint64_t)i, A.L, A.base, A.w) : A.S.keys[i];
   ST dr = 0, di = 0, orr[NV], oi[NV];
@@ -56,8 +76,15 @@ __global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *
       int64_t delta = 0;
       int cc = c;
       for (int q = ns - 1; q >= 0; q--) {
-        uint64_t nd = (uint64_t)(cc % A.base);
-        cc /= A.base;
+        // the local column's digits, last site first (base 2: its bits)
+        uint64_t nd;
+        if (A.base == 2) {
+          nd = (uint64_t)(cc & 1);
+          cc >>= 1;
+        } else {
+          nd = (uint64_t)(cc % A.base);
+          cc /= A.base;
+        }
         int sh = td[2 + q];
         uint64_t od = (s2 >> sh) & fm;
         s2 = (s2 & ~(fm << sh)) | (nd << sh);
@@ -109,25 +136,34 @@ __global__ void __launch_bounds__(256) k_matvec_generic(MatvecArgs A, const VT *
   }
 }
 
-template <typename VT, typename ST>
-static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, int accumulate, hipStream_t st) {
+template <typename VT, typename ST, bool STAGED>
+static void launch_generic_t(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, int accumulate, hipStream_t st) {
   unsigned grid = (unsigned)((a.D + 255) / 256);
   for (int64_t c = 0; c < nb;) {
     const int64_t left = nb - c;
     if (left >= 8) {
-      k_matvec_generic<VT, ST, 8><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      k_matvec_generic<VT, ST, 8, STAGED><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
       c += 8;
     } else if (left >= 4) {
-      k_matvec_generic<VT, ST, 4><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      k_matvec_generic<VT, ST, 4, STAGED><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
       c += 4;
     } else if (left >= 2) {
-      k_matvec_generic<VT, ST, 2><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      k_matvec_generic<VT, ST, 2, STAGED><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
       c += 2;
     } else {
-      k_matvec_generic<VT, ST, 1><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
+      k_matvec_generic<VT, ST, 1, STAGED><<<grid, 256, 0, st>>>(a, x, y, nb, c, accumulate);
       c += 1;
     }
   }
+}
+
+template <typename VT, typename ST>
+static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, int accumulate, hipStream_t st) {
+  const TermsDev &T = a.T;
+  if (T.nmo <= GEN_LDS_MO && T.nrp <= GEN_LDS_RP && T.nnz <= GEN_LDS_NZ)
+    launch_generic_t<VT, ST, true>(a, x, y, nb, accumulate, st);
+  else
+    launch_generic_t<VT, ST, false>(a, x, y, nb, accumulate, st);
 }
 
 // count of off-diagonal targets present in the sector (the NNZ_off figure)
@@ -303,6 +339,9 @@ extern "C" int edk_operator_create(edk_ctx *ctx, edk_basis *b, int64_t nterms, c
   op->terms.rowptr = op->d_rowptr;
   op->terms.col = op->d_col;
   op->terms.val = op->d_val;
+  op->terms.nmo = (int)mrow_off.size();
+  op->terms.nrp = (int)rowptr.size();
+  op->terms.nnz = (int)col.size();
   b->refcount++;
   *out = op;
   return EDK_OK;
