@@ -657,7 +657,8 @@ def c6_workload(args, ev):
 def c7_workload(args, ev):
     """Custom generators: nearest-neighbour Heisenberg model on a 6 x 5 torus
     (L = 30), momentum (0, 0) with T_x and T_y given as site permutations
-    (basis(...; symmetries), applied as Benes delta-swap networks), N = 15.
+    (basis(...; symmetries); T_x is applied in the shift-and-mask form, T_y as
+    a rotation of the packed word), N = 15.
     One step = one ComplexF64 H*psi through the generic gather kernel;
     roofline bytes as for the default line (D (8 + 8 + 32) + NNZ_off (16 + 8)).
     The CPU leg times the oracle's threaded scatter on a column sample."""

@@ -777,7 +777,11 @@ static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, in
   else launch_fast_lt<KIND, PH, 0>(A, dtype, y, x, acc, st);
 }
 
-bool edk_matvec_prepare(edk_op *op) { return !op->disable_fast && fast_setup(op); }
+bool edk_matvec_prepare(edk_op *op) {
+  const bool f = !op->disable_fast && fast_setup(op);
+  if (!f) (void)edk_generic_setup(op);
+  return f;
+}
 
 // Launch one schedule: EDK_VARIANT_TILE is the balanced chunk kernel (one
 // block per chunk of whole rows), EDK_VARIANT_LANE{2,4,8} the thread-per-row

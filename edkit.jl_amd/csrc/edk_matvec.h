@@ -40,6 +40,9 @@ struct edk_op {
   void *fast = nullptr;       // state of the specialised kernel (edk_fast.hip)
   void *csr = nullptr;        // assembled sector matrix (edk_assembly.hip), or null
   bool disable_fast = false;  // force the generic kernel (tests compare both)
+  uint32_t *d_gchunk = nullptr;  // generic tile kernel: first row of each chunk
+  uint32_t ngchunk = 0;
+  int gstate = 0;                 // generic tile setup: 0 not yet, 1 built, -1 not applicable
   size_t scratch_bytes = 0;
   std::vector<int> h_tdesc, h_mrow_off, h_rowptr, h_col;
   std::vector<double2> h_val;
@@ -67,6 +70,8 @@ int edk_fast_set_variant(edk_op *op, int dtype, int64_t nb, int variant);
 int edk_fast_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant, double *ms);
 // one-time setup of the specialised kernels (allocates; run before capture)
 bool edk_matvec_prepare(edk_op *op);
+// one-time setup of the generic tile kernel (allocates; run before capture)
+bool edk_generic_setup(edk_op *op);
 // y = (accumulate ? y : 0) + H x on stream st (no allocation, no synchronisation)
 int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);
 inline int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {

@@ -15,8 +15,10 @@
 // written once by one lane: no atomics, and the summation order per row is
 // fixed, so results are bitwise reproducible run to run.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "edk_matvec.h"
@@ -164,6 +166,296 @@ static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, 
     launch_generic_t<VT, ST, true>(a, x, y, nb, accumulate, st);
   else
     launch_generic_t<VT, ST, false>(a, x, y, nb, accumulate, st);
+}
+
+// ---------------------------------------------------------------------------
+// generic tile kernel (one vector).  In the one-lane-per-row kernel a lane
+// whose row has no target at a term idles while the others canonicalise, so
+// half the lanes idle for a two-site spin exchange.  Here the targets are
+// compacted into LDS first and every lane of the block canonicalises them.
+// One block per chunk of whole rows holding at most GT_CAP off-diagonal local
+// entries (chunk: first row of each chunk, built once per operator):
+//   phase 1: row lanes walk their terms; diagonal entries stay in registers,
+//            off-diagonal images b' and their entry ids go to LDS at the row's
+//            offset from a block scan, in term and entry order;
+//   phase 2: lane tid takes targets tid + GT_TPB q: canonical form and lookup
+//            (position over the image, coefficient into its contribution
+//            slot), then the GT_K x gathers together;
+//   phase 3: row lanes sum their slice in order, add the diagonal, scale by
+//            1/R_i and write y.
+// The per-row summation order is the row kernel's; no atomics.
+// ---------------------------------------------------------------------------
+#define GT_TPB 256
+#define GT_K 4
+#define GT_CAP (GT_TPB * GT_K)
+#define GT_NONE (~0ull)
+#define GT_MAXROWS (1ll << 21)
+
+__device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, uint32_t *wsum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t xx = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t yv = __shfl_up(xx, o, 64);
+    if (lane >= o) xx += yv;
+  }
+  if (lane == 63) wsum[wid] = xx;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int q = 0; q < GT_TPB / 64; q++) {
+    if (q < wid) base += wsum[q];
+    tot += wsum[q];
+  }
+  *total = tot;
+  return base + xx - v;
+}
+
+template <typename VT, typename ST>
+__global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
+                                                         const VT *__restrict__ x, VT *__restrict__ y,
+                                                         int accumulate) {
+  __shared__ uint64_t word[GT_CAP];  // image b', then its position (GT_NONE: absent)
+  __shared__ uint16_t ent[GT_CAP];   // entry id into T.col / T.val
+  __shared__ VT contr[GT_CAP];       // coefficient, then contribution
+  __shared__ uint32_t wsum[GT_TPB / 64];
+  const TermsDev &T = A.T;
+  const int tid = threadIdx.x;
+  const uint32_t r0 = chunk[blockIdx.x], nrow = chunk[blockIdx.x + 1] - r0;
+  const bool own = (uint32_t)tid < nrow;
+  const uint64_t fm = (1ull << A.w) - 1;
+  uint64_t s = 0;
+  uint32_t n = 0;
+  ST dr = 0, di = 0;
+  VT xi{};
+  ST invR = 1;
+  if (own) {
+    const int64_t i = (int64_t)r0 + tid;
+    s = A.S.keys[i];
+    xi = x[i];
+    if (A.kind != EDK_PROJECTED) invR = (ST)(1.0 / A.R[i]);
+    for (int t = 0; t < T.nterms; t++) {
+      const int *td = T.tdesc + t * EDK_TDESC;
+      const int ns = td[0], m = td[1];
+      int loc = 0;
+      for (int q = 0; q < ns; q++) loc = loc * A.base + (int)((s >> td[2 + q]) & fm);
+      const int *rp = T.rowptr + T.mrow_off[m];
+      for (int e = rp[loc]; e < rp[loc + 1]; e++) {
+        if (T.col[e] == loc) {
+          const double2 v = T.val[e];
+          dr += (ST)v.x;
+          di += (ST)v.y;
+        } else {
+          n++;
+        }
+      }
+    }
+  }
+  uint32_t total;
+  const uint32_t off = gt_block_scan(n, &total, wsum);
+  if (own) {
+    uint32_t o = off;
+    for (int t = 0; t < T.nterms; t++) {
+      const int *td = T.tdesc + t * EDK_TDESC;
+      const int ns = td[0], m = td[1];
+      int loc = 0;
+      for (int q = 0; q < ns; q++) loc = loc * A.base + (int)((s >> td[2 + q]) & fm);
+      const int *rp = T.rowptr + T.mrow_off[m];
+      for (int e = rp[loc]; e < rp[loc + 1]; e++) {
+        int cc = T.col[e];
+        if (cc == loc) continue;
+        uint64_t s2 = s;
+        for (int q = ns - 1; q >= 0; q--) {
+          uint64_t nd;
+          if (A.base == 2) {
+            nd = (uint64_t)(cc & 1);
+            cc >>= 1;
+          } else {
+            nd = (uint64_t)(cc % A.base);
+            cc /= A.base;
+          }
+          const int sh = td[2 + q];
+          s2 = (s2 & ~(fm << sh)) | (nd << sh);
+        }
+        word[o] = s2;
+        ent[o] = (uint16_t)e;
+        o++;
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2a: canonical form and lookup, one target at a time; each slot is
+  // read and rewritten by its own lane only
+#pragma unroll 1
+  for (int q = 0; q < GT_K; q++) {
+    const uint32_t j = tid + GT_TPB * q;
+    if (j < total) {
+      const uint64_t t = word[j];
+      int64_t jx;
+      double cr = 1.0, ci = 0.0;
+      if (A.kind == EDK_PROJECTED) {
+        jx = edk_search(A.S, t);
+      } else {
+        const CanonOut o = edk_canon(A.C, t);
+        jx = (A.orbit_norms && !o.ok) ? -1 : edk_search(A.S, o.rep);
+        if (jx >= 0) {
+          const double Rj = A.orbit_norms ? o.R : A.R[jx];
+          cr = o.phase.x * Rj;  // conj(phase) * R_j
+          ci = -o.phase.y * Rj;
+        }
+      }
+      const double2 v = T.val[ent[j]];
+      VT c;
+      c.x = (ST)(v.x * cr - v.y * ci);
+      c.y = (ST)(v.x * ci + v.y * cr);
+      contr[j] = c;
+      word[j] = jx < 0 ? GT_NONE : (uint64_t)jx;
+    }
+  }
+  // phase 2b: the lane's gathers in flight together; an absent target reads
+  // x[0] and is dropped by selection
+  uint64_t p[GT_K];
+  VT xg[GT_K];
+#pragma unroll
+  for (int q = 0; q < GT_K; q++) {
+    const uint32_t j = tid + GT_TPB * q;
+    p[q] = j < total ? word[j] : GT_NONE;
+  }
+#pragma unroll
+  for (int q = 0; q < GT_K; q++) xg[q] = x[p[q] != GT_NONE ? p[q] : 0];
+#pragma unroll
+  for (int q = 0; q < GT_K; q++) {
+    const uint32_t j = tid + GT_TPB * q;
+    if (j < total) {
+      const VT c = contr[j];
+      VT cv;
+      cv.x = c.x * (ST)xg[q].x - c.y * (ST)xg[q].y;
+      cv.y = c.x * (ST)xg[q].y + c.y * (ST)xg[q].x;
+      if (p[q] == GT_NONE) cv.x = cv.y = 0;
+      contr[j] = cv;
+    }
+  }
+  __syncthreads();
+  if (own) {
+    const int64_t i = (int64_t)r0 + tid;
+    ST orr = 0, oi = 0;
+    for (uint32_t q = off; q < off + n; q++) {
+      const VT cv = contr[q];
+      orr += cv.x;
+      oi += cv.y;
+    }
+    const ST yr = dr * (ST)xi.x - di * (ST)xi.y + orr * invR;
+    const ST yi = dr * (ST)xi.y + di * (ST)xi.x + oi * invR;
+    VT out;
+    if (accumulate) {
+      const VT y0 = y[i];
+      out.x = y0.x + yr;
+      out.y = y0.y + yi;
+    } else {
+      out.x = yr;
+      out.y = yi;
+    }
+    y[i] = out;
+  }
+}
+
+// off-diagonal local entries per row (targets before the lookup), for chunking
+__global__ void __launch_bounds__(256) k_row_entries(MatvecArgs A, uint32_t *cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.D) return;
+  const TermsDev &T = A.T;
+  const uint64_t fm = (1ull << A.w) - 1;
+  const uint64_t s = A.S.keys[i];
+  uint32_t n = 0;
+  for (int t = 0; t < T.nterms; t++) {
+    const int *td = T.tdesc + t * EDK_TDESC;
+    const int ns = td[0], m = td[1];
+    int loc = 0;
+    for (int q = 0; q < ns; q++) loc = loc * A.base + (int)((s >> td[2 + q]) & fm);
+    const int *rp = T.rowptr + T.mrow_off[m];
+    for (int e = rp[loc]; e < rp[loc + 1]; e++) n += T.col[e] != loc;
+  }
+  cnt[i] = n;
+}
+
+static std::mutex g_gtile_mu;  // first-time setup may be reached from several threads
+
+// Chunks of whole rows with at most GT_CAP off-diagonal entries and GT_TPB
+// rows each.  Not applicable: tensor bases (no table), more than 2^32 - 2
+// rows, entry ids beyond 16 bits, a row with more than GT_CAP entries.
+// Used by default for sectors of fewer than GT_MAXROWS rows that need a
+// canonical form (not ProjectedBasis).  There one lane per row leaves most of
+// the chip idle (5 x 5 torus at k = (1, 2), D = 208 012: 0.46 against 0.88 ms;
+// a translation-parity ring at D = 718 146: 0.56 against 0.72 ms).  In larger
+// sectors the row kernel is faster: its per-term table walk, a chain of
+// dependent loads, runs on every lane, where here one wave per chunk walks it
+// while the others wait (6 x 5 torus: 14.4 against 13.5 ms; a projected
+// J1-J2 ring at D = 10.4 M: 13.6 against 3.7 ms).  EDK_GENERIC_SCHEDULE=row
+// or =tile overrides the choice.  Allocates and synchronises: run before
+// capture.
+bool edk_generic_setup(edk_op *op) {
+  std::lock_guard<std::mutex> lk(g_gtile_mu);
+  if (op->gstate) return op->gstate > 0;
+  op->gstate = -1;
+  const edk_basis *b = op->basis;
+  const char *env = getenv("EDK_GENERIC_SCHEDULE");
+  if (env && strcmp(env, "row") == 0) return false;
+  const bool force = env && strcmp(env, "tile") == 0;
+  if (!force && (b->kind == EDK_PROJECTED || b->D >= GT_MAXROWS)) return false;
+  if (b->kind == EDK_TENSOR || b->D == 0 || b->D >= (1ll << 32) - 1 || op->terms.nnz > 65535) return false;
+  edk_ctx *ctx = op->ctx;
+  uint32_t *d_cnt = nullptr;
+  std::vector<uint32_t> cnt(b->D);
+  if (hipMalloc((void **)&d_cnt, b->D * sizeof(uint32_t)) != hipSuccess) return false;
+  MatvecArgs a = op->args();
+  k_row_entries<<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(a, d_cnt);
+  const bool okc = hipGetLastError() == hipSuccess &&
+                   hipMemcpyAsync(cnt.data(), d_cnt, b->D * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream) ==
+                       hipSuccess &&
+                   hipStreamSynchronize(ctx->stream) == hipSuccess;
+  (void)hipFree(d_cnt);
+  if (!okc) return false;
+  std::vector<uint32_t> cs;
+  cs.push_back(0);
+  uint32_t acc = 0, rows = 0;
+  for (int64_t i = 0; i < b->D; i++) {
+    if (cnt[i] > GT_CAP) return false;
+    if (rows == GT_TPB || acc + cnt[i] > GT_CAP) {
+      cs.push_back((uint32_t)i);
+      acc = 0;
+      rows = 0;
+    }
+    acc += cnt[i];
+    rows++;
+  }
+  cs.push_back((uint32_t)b->D);
+  if (hipMalloc((void **)&op->d_gchunk, cs.size() * sizeof(uint32_t)) != hipSuccess) return false;
+  if (hipMemcpyAsync(op->d_gchunk, cs.data(), cs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    (void)hipFree(op->d_gchunk);
+    op->d_gchunk = nullptr;
+    return false;
+  }
+  op->ngchunk = (uint32_t)cs.size() - 1;
+  op->gstate = 1;
+  return true;
+}
+
+// the tile form when it is set up (or can be: never under stream capture)
+static bool launch_gtile(edk_op *op, int dtype, void *y, const void *x, int accumulate, hipStream_t st) {
+  if (op->gstate == 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    if (!edk_generic_setup(op)) return false;
+  }
+  if (op->gstate < 0) return false;
+  const MatvecArgs a = op->args();
+  if (dtype == EDK_C64)
+    k_matvec_gtile<double2, double><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
+                                                                    (double2 *)y, accumulate);
+  else
+    k_matvec_gtile<float2, float><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
+                                                                 (float2 *)y, accumulate);
+  return true;
 }
 
 // count of off-diagonal targets present in the sector (the NNZ_off figure)
@@ -358,6 +650,7 @@ extern "C" int edk_operator_destroy(edk_op *op) {
   hipFree(op->d_col);
   hipFree(op->d_val);
   hipFree(op->d_scratch);
+  hipFree(op->d_gchunk);
   edk_fast_release(op);
   edk_csr_release(op);
   if (op->basis) op->basis->refcount--;
@@ -388,6 +681,10 @@ int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
   // several right-hand sides with the assembled matrix cached (sparse!)
   if (nb > 1 && edk_matvec_csr(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
+  if (nb == 1 && launch_gtile(op, dtype, y, x, accumulate, st)) {
+    EDK_HIP(hipGetLastError());
+    return EDK_OK;
+  }
   if (dtype == EDK_C64)
     launch_generic<double2, double>(a, (const double2 *)x, (double2 *)y, nb, accumulate, st);
   else

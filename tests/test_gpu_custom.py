@@ -1,7 +1,7 @@
 """Custom permutation generators on the device (basis(...; symmetries),
 AbelianBasis.jl:771-847, the 2D example at :754-760): 2D tori with T_x and
-T_y given as site permutations (applied on the device as Benes delta-swap
-networks), with and without a custom spin flip.  Tables, R bits and index()
+T_y given as site permutations (T_x in the shift-and-mask form, T_y as a
+rotation of the packed word), with and without a custom spin flip.  Tables, R bits and index()
 bit-exact against the oracle; the XXZ matvec to 1e-12 and the Lanczos ground
 energy against the dense sector matrix."""
 import numpy as np
