@@ -304,27 +304,30 @@ __device__ __forceinline__ GenReg edk_load_gen(const CanonDev &C, int g) {
   return r;
 }
 
-// edk_apply_gen for a register-resident generator
-__device__ __forceinline__ uint64_t edk_apply_genreg(const CanonDev &C, const GenReg &r, uint64_t s) {
+// edk_apply_gen for a register-resident generator, on a word type W that
+// holds the L w-bit fields (uint32_t when L w <= 32: one instruction per
+// logical operation instead of two)
+template <typename W>
+__device__ __forceinline__ W edk_apply_genreg(const CanonDev &C, const GenReg &r, W s) {
   if (r.type == GEN_ROTL) {
     const int tot = C.L * C.w;
-    if (r.rot) s = ((s << r.rot) | (s >> (tot - r.rot))) & C.mask;
+    if (r.rot) s = ((s << r.rot) | (s >> (tot - r.rot))) & (W)C.mask;
   } else if (r.type == GEN_REV) {
-    s = edk_revfields(s, C.L, C.w);
+    s = (W)edk_revfields((uint64_t)s, C.L, C.w);
   } else if (r.type == GEN_SHIFTS) {
-    uint64_t o = 0;
+    W o = 0;
 #pragma unroll
     for (int q = 0; q < EDK_MAXSHIFTS; q++) {
       if (q < r.n) {
-        const uint64_t m = s & r.m[q];
-        o |= r.sh[q] >= 0 ? (m << r.sh[q]) : (m >> -r.sh[q]);
+        const W m = s & (W)r.m[q];
+        o |= r.sh[q] >= 0 ? (W)(m << r.sh[q]) : (W)(m >> -r.sh[q]);
       }
     }
     s = o;
   }
   if (r.inv) {
-    if (C.base != 3) s ^= r.inv;
-    else s ^= r.inv & ~((s & C.lowfields) << 1);
+    if (C.base != 3) s ^= (W)r.inv;
+    else s ^= (W)r.inv & (W) ~((s & (W)C.lowfields) << 1);
   }
   return s;
 }
@@ -334,33 +337,35 @@ __device__ __forceinline__ uint64_t edk_apply_genreg(const CanonDev &C, const Ge
 // (generator 0 fastest, the odometer order), so the loop control is uniform
 // and each element costs one generator application; the first strict
 // minimum is kept as its element number e = a0 + n0 (a1 + n1 a2).
-__device__ __forceinline__ CanonOut edk_canon_abelian3(const CanonDev &C, uint64_t s) {
+template <typename W>
+__device__ __forceinline__ CanonOut edk_canon_abelian3(const CanonDev &C, uint64_t s64) {
   const GenReg g0 = edk_load_gen(C, 0);
   const GenReg g1 = C.ngen > 1 ? edk_load_gen(C, 1) : g0;
   const GenReg g2 = C.ngen > 2 ? edk_load_gen(C, 2) : g0;
   const int n0 = g0.order, n1 = C.ngen > 1 ? g1.order : 1, n2 = C.ngen > 2 ? g2.order : 1;
-  uint64_t best = s;
+  const W s = (W)s64;
+  W best = s;
   int be = 0, nstab = 0, e = 0;
-  uint64_t c2 = s;
+  W c2 = s;
   for (int a2 = 0; a2 < n2; a2++) {
-    uint64_t c1 = c2;
+    W c1 = c2;
     for (int a1 = 0; a1 < n1; a1++) {
-      uint64_t c0 = c1;
+      W c0 = c1;
       for (int a0 = 0; a0 < n0; a0++, e++) {
         nstab += c0 == s;
         if (c0 < best) {
           best = c0;
           be = e;
         }
-        if (a0 + 1 < n0) c0 = edk_apply_genreg(C, g0, c0);
+        if (a0 + 1 < n0) c0 = edk_apply_genreg<W>(C, g0, c0);
       }
-      if (a1 + 1 < n1) c1 = edk_apply_genreg(C, g1, c1);
+      if (a1 + 1 < n1) c1 = edk_apply_genreg<W>(C, g1, c1);
     }
-    if (a2 + 1 < n2) c2 = edk_apply_genreg(C, g2, c2);
+    if (a2 + 1 < n2) c2 = edk_apply_genreg<W>(C, g2, c2);
   }
   const int m0 = be % n0 + 1, m1 = (be / n0) % n1 + 1, m2 = be / (n0 * n1) + 1;
   CanonOut o;
-  o.rep = best;
+  o.rep = (uint64_t)best;
   o.ok = 1;
   o.R = C.Rtab[nstab - 1];
   double2 acc = C.ph[C.gdesc[6] + m0 - 1];
@@ -373,7 +378,7 @@ __device__ __forceinline__ CanonOut edk_canon_abelian3(const CanonDev &C, uint64
 __device__ __forceinline__ CanonOut edk_canon_abelian(const CanonDev &C, uint64_t s) {
   if (C.ngen >= 1 && C.ngen <= 3 && C.gdesc[0] != GEN_BENES && (C.ngen < 2 || C.gdesc[EDK_GDESC] != GEN_BENES) &&
       (C.ngen < 3 || C.gdesc[2 * EDK_GDESC] != GEN_BENES))
-    return edk_canon_abelian3(C, s);
+    return C.L * C.w <= 32 ? edk_canon_abelian3<uint32_t>(C, s) : edk_canon_abelian3<uint64_t>(C, s);
   int os[EDK_MAXGEN], ms[EDK_MAXGEN];
   for (int g = 0; g < C.ngen; g++) os[g] = ms[g] = 1;
   uint64_t best = s, cur = s;

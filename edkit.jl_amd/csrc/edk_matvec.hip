@@ -175,21 +175,23 @@ static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, 
 // compacted into LDS first and every lane of the block canonicalises them.
 // One block per chunk of whole rows holding at most GT_CAP off-diagonal local
 // entries (chunk: first row of each chunk, built once per operator):
-//   phase 1: row lanes walk their terms; diagonal entries stay in registers,
-//            off-diagonal images b' and their entry ids go to LDS at the row's
-//            offset from a block scan, in term and entry order;
+//   phase 1: GT_Q lanes per row walk a share of its terms each, over term
+//            tables staged in LDS; diagonal entries stay in registers,
+//            off-diagonal images b' and their entry ids go to LDS at the
+//            lane's offset from a block scan, so each row's targets lie in
+//            term and entry order;
 //   phase 2: lane tid takes targets tid + GT_TPB q: canonical form and lookup
 //            (position over the image, coefficient into its contribution
 //            slot), then the GT_K x gathers together;
-//   phase 3: row lanes sum their slice in order, add the diagonal, scale by
-//            1/R_i and write y.
-// The per-row summation order is the row kernel's; no atomics.
+//   phase 3: each row's first lane adds its shares' diagonal sums, sums the
+//            row's slice in order, scales by 1/R_i and writes y.
+// A row's off-diagonal sum runs in the row kernel's order; no atomics.
 // ---------------------------------------------------------------------------
 #define GT_TPB 256
 #define GT_K 4
 #define GT_CAP (GT_TPB * GT_K)
 #define GT_NONE (~0ull)
-#define GT_MAXROWS (1ll << 21)
+#define GT_Q 4  // lanes per row in phase 1
 
 __device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, uint32_t *wsum) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -209,7 +211,7 @@ __device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, u
   return base + xx - v;
 }
 
-template <typename VT, typename ST>
+template <typename VT, typename ST, bool STAGED>
 __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
                                                          const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate) {
@@ -217,22 +219,38 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
   __shared__ uint16_t ent[GT_CAP];   // entry id into T.col / T.val
   __shared__ VT contr[GT_CAP];       // coefficient, then contribution
   __shared__ uint32_t wsum[GT_TPB / 64];
-  const TermsDev &T = A.T;
+  __shared__ int s_mo[STAGED ? GEN_LDS_MO : 1], s_rp[STAGED ? GEN_LDS_RP : 1], s_col[STAGED ? GEN_LDS_NZ : 1];
+  __shared__ double2 s_val[STAGED ? GEN_LDS_NZ : 1];
+  TermsDev T = A.T;
   const int tid = threadIdx.x;
+  if constexpr (STAGED) {
+    for (int k = tid; k < T.nmo; k += GT_TPB) s_mo[k] = T.mrow_off[k];
+    for (int k = tid; k < T.nrp; k += GT_TPB) s_rp[k] = T.rowptr[k];
+    for (int k = tid; k < T.nnz; k += GT_TPB) {
+      s_col[k] = T.col[k];
+      s_val[k] = T.val[k];
+    }
+    __syncthreads();
+    T.mrow_off = s_mo;
+    T.rowptr = s_rp;
+    T.col = s_col;
+    T.val = s_val;
+  }
+  // GT_Q lanes per row (consecutive lanes of one wave), each walking one
+  // share of the terms; the block scan in lane order then lays each row's
+  // targets out in term order
   const uint32_t r0 = chunk[blockIdx.x], nrow = chunk[blockIdx.x + 1] - r0;
-  const bool own = (uint32_t)tid < nrow;
+  const uint32_t row = (uint32_t)tid / GT_Q;
+  const int qtr = tid % GT_Q;
+  const bool own = row < nrow;
+  const int t0 = (qtr * T.nterms) / GT_Q, t1 = ((qtr + 1) * T.nterms) / GT_Q;
   const uint64_t fm = (1ull << A.w) - 1;
   uint64_t s = 0;
   uint32_t n = 0;
   ST dr = 0, di = 0;
-  VT xi{};
-  ST invR = 1;
   if (own) {
-    const int64_t i = (int64_t)r0 + tid;
-    s = A.S.keys[i];
-    xi = x[i];
-    if (A.kind != EDK_PROJECTED) invR = (ST)(1.0 / A.R[i]);
-    for (int t = 0; t < T.nterms; t++) {
+    s = A.S.keys[(int64_t)r0 + row];
+    for (int t = t0; t < t1; t++) {
       const int *td = T.tdesc + t * EDK_TDESC;
       const int ns = td[0], m = td[1];
       int loc = 0;
@@ -253,7 +271,7 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
   const uint32_t off = gt_block_scan(n, &total, wsum);
   if (own) {
     uint32_t o = off;
-    for (int t = 0; t < T.nterms; t++) {
+    for (int t = t0; t < t1; t++) {
       const int *td = T.tdesc + t * EDK_TDESC;
       const int ns = td[0], m = td[1];
       int loc = 0;
@@ -334,16 +352,29 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
     }
   }
   __syncthreads();
-  if (own) {
-    const int64_t i = (int64_t)r0 + tid;
+  // phase 3: the first lane of each row gathers its shares' counts and
+  // diagonal sums (share order), then sums the row's slice in order
+  const int lead = (tid & 63) - qtr;  // the row's first lane within the wave
+  uint32_t nr = 0;
+  ST dsr = 0, dsi = 0;
+#pragma unroll
+  for (int q = 0; q < GT_Q; q++) {
+    nr += __shfl(n, lead + q, 64);
+    dsr += __shfl(dr, lead + q, 64);
+    dsi += __shfl(di, lead + q, 64);
+  }
+  if (own && qtr == 0) {
+    const int64_t i = (int64_t)r0 + row;
+    const VT xi = x[i];
+    const ST invR = A.kind != EDK_PROJECTED ? (ST)(1.0 / A.R[i]) : (ST)1;
     ST orr = 0, oi = 0;
-    for (uint32_t q = off; q < off + n; q++) {
+    for (uint32_t q = off; q < off + nr; q++) {
       const VT cv = contr[q];
       orr += cv.x;
       oi += cv.y;
     }
-    const ST yr = dr * (ST)xi.x - di * (ST)xi.y + orr * invR;
-    const ST yi = dr * (ST)xi.y + di * (ST)xi.x + oi * invR;
+    const ST yr = dsr * (ST)xi.x - dsi * (ST)xi.y + orr * invR;
+    const ST yi = dsr * (ST)xi.y + dsi * (ST)xi.x + oi * invR;
     VT out;
     if (accumulate) {
       const VT y0 = y[i];
@@ -378,19 +409,17 @@ __global__ void __launch_bounds__(256) k_row_entries(MatvecArgs A, uint32_t *cnt
 
 static std::mutex g_gtile_mu;  // first-time setup may be reached from several threads
 
-// Chunks of whole rows with at most GT_CAP off-diagonal entries and GT_TPB
-// rows each.  Not applicable: tensor bases (no table), more than 2^32 - 2
-// rows, entry ids beyond 16 bits, a row with more than GT_CAP entries.
-// Used by default for sectors of fewer than GT_MAXROWS rows that need a
-// canonical form (not ProjectedBasis).  There one lane per row leaves most of
-// the chip idle (5 x 5 torus at k = (1, 2), D = 208 012: 0.46 against 0.88 ms;
-// a translation-parity ring at D = 718 146: 0.56 against 0.72 ms).  In larger
-// sectors the row kernel is faster: its per-term table walk, a chain of
-// dependent loads, runs on every lane, where here one wave per chunk walks it
-// while the others wait (6 x 5 torus: 14.4 against 13.5 ms; a projected
-// J1-J2 ring at D = 10.4 M: 13.6 against 3.7 ms).  EDK_GENERIC_SCHEDULE=row
-// or =tile overrides the choice.  Allocates and synchronises: run before
-// capture.
+// Chunks of whole rows with at most GT_CAP off-diagonal entries and
+// GT_TPB / GT_Q rows each.  Not applicable: tensor bases (no table), more than
+// 2^32 - 2 rows, entry ids beyond 16 bits, a row with more than GT_CAP
+// entries.  Used by default for every basis that needs a canonical form, where
+// it beat the row kernel at every size measured (ComplexF64, row -> tile):
+// 5 x 5 torus at k = (1, 2), D = 208 012: 0.88 -> 0.39 ms; 6 x 5 torus,
+// D = 5.17 M: 13.5 -> 13.1 ms.  ProjectedBasis keeps the row kernel: with no
+// canonical form to compute, the row kernel's lanes look up the same term's
+// images of consecutive rows, which share cache lines, and it is faster
+// (J1-J2 ring, D = 10.4 M: 3.7 against 9.5 ms).  EDK_GENERIC_SCHEDULE=row or
+// =tile overrides the choice.  Allocates and synchronises: run before capture.
 bool edk_generic_setup(edk_op *op) {
   std::lock_guard<std::mutex> lk(g_gtile_mu);
   if (op->gstate) return op->gstate > 0;
@@ -399,7 +428,7 @@ bool edk_generic_setup(edk_op *op) {
   const char *env = getenv("EDK_GENERIC_SCHEDULE");
   if (env && strcmp(env, "row") == 0) return false;
   const bool force = env && strcmp(env, "tile") == 0;
-  if (!force && (b->kind == EDK_PROJECTED || b->D >= GT_MAXROWS)) return false;
+  if (!force && b->kind == EDK_PROJECTED) return false;
   if (b->kind == EDK_TENSOR || b->D == 0 || b->D >= (1ll << 32) - 1 || op->terms.nnz > 65535) return false;
   edk_ctx *ctx = op->ctx;
   uint32_t *d_cnt = nullptr;
@@ -418,7 +447,7 @@ bool edk_generic_setup(edk_op *op) {
   uint32_t acc = 0, rows = 0;
   for (int64_t i = 0; i < b->D; i++) {
     if (cnt[i] > GT_CAP) return false;
-    if (rows == GT_TPB || acc + cnt[i] > GT_CAP) {
+    if (rows == GT_TPB / GT_Q || acc + cnt[i] > GT_CAP) {
       cs.push_back((uint32_t)i);
       acc = 0;
       rows = 0;
@@ -449,12 +478,23 @@ static bool launch_gtile(edk_op *op, int dtype, void *y, const void *x, int accu
   }
   if (op->gstate < 0) return false;
   const MatvecArgs a = op->args();
-  if (dtype == EDK_C64)
-    k_matvec_gtile<double2, double><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
-                                                                    (double2 *)y, accumulate);
-  else
-    k_matvec_gtile<float2, float><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
-                                                                 (float2 *)y, accumulate);
+  const TermsDev &T = a.T;
+  const bool staged = T.nmo <= GEN_LDS_MO && T.nrp <= GEN_LDS_RP && T.nnz <= GEN_LDS_NZ;
+  if (dtype == EDK_C64) {
+    if (staged)
+      k_matvec_gtile<double2, double, true><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
+                                                                          (double2 *)y, accumulate);
+    else
+      k_matvec_gtile<double2, double, false><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
+                                                                           (double2 *)y, accumulate);
+  } else {
+    if (staged)
+      k_matvec_gtile<float2, float, true><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
+                                                                        (float2 *)y, accumulate);
+    else
+      k_matvec_gtile<float2, float, false><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
+                                                                         (float2 *)y, accumulate);
+  }
   return true;
 }
 
