@@ -312,6 +312,14 @@ __global__ void __launch_bounds__(1024) k_scan_counts(const uint64_t *cnt, int64
   }
 }
 
+// largest directory slot (integer maximum: order-independent)
+__global__ void k_dir_maxgap(const uint32_t *dir, int64_t ndir, unsigned int *out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned int g = b < ndir ? dir[b + 1] - dir[b] : 0u;
+  for (int o = 32; o > 0; o >>= 1) g = max(g, (unsigned int)__shfl_down(g, o, 64));
+  if ((threadIdx.x & 63) == 0 && g) atomicMax(out, g);
+}
+
 // directory: dir[b] = lower_bound(keys, b << dshift), dir[ndir] = D
 __global__ void k_build_dir(const uint64_t *keys, int64_t D, int dshift, int64_t ndir, uint32_t *dir) {
   int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,6 +409,7 @@ SearchDev edk_basis::search() const {
   s.dir = d_dir;
   s.dshift = dshift;
   s.ndir = ndir;
+  s.nprobe = nprobe;
   return s;
 }
 
@@ -719,7 +728,23 @@ static int build_directory(edk_basis *b) {
   int64_t nthr = b->ndir + 1;
   k_build_dir<<<(unsigned)((nthr + 255) / 256), 256, 0, ctx->stream>>>(b->d_keys, b->D, sh, b->ndir, b->d_dir);
   EDK_HIP(hipGetLastError());
-  EDK_HIP(hipStreamSynchronize(ctx->stream));
+  // halving steps of a lower-bound search over the fullest slot: floor(log2 n) + 1
+  unsigned int *d_gap = nullptr, gap = 0;
+  EDK_HIP(hipMalloc((void **)&d_gap, sizeof(unsigned int)));
+  int rc = EDK_OK;
+  if (hipMemsetAsync(d_gap, 0, sizeof(unsigned int), ctx->stream) != hipSuccess) rc = edk_fail(EDK_ERR_HIP, "directory");
+  if (!rc) {
+    k_dir_maxgap<<<(unsigned)((b->ndir + 255) / 256), 256, 0, ctx->stream>>>(b->d_dir, b->ndir, d_gap);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&gap, d_gap, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = edk_fail(EDK_ERR_HIP, "directory");
+  }
+  (void)hipFree(d_gap);
+  if (rc) return rc;
+  int np = 0;
+  while (gap >> np) np++;
+  b->nprobe = np;
   return EDK_OK;
 }
 

@@ -84,6 +84,7 @@ struct SearchDev {
   const uint32_t *dir;   // dir[b] = first index with key >= (b << dshift), b in [0, ndir]
   int dshift;
   int64_t ndir;
+  int nprobe;            // halving steps that resolve the fullest directory slot
 };
 
 struct edk_basis {
@@ -98,6 +99,7 @@ struct edk_basis {
   uint32_t *d_dir = nullptr;
   int dshift = 0;
   int64_t ndir = 0;
+  int nprobe = 0;
   // canonicaliser tables
   double *d_Rtab = nullptr;
   double2 *d_ph = nullptr;

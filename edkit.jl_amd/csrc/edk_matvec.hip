@@ -175,14 +175,12 @@ static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, 
 // compacted into LDS first and every lane of the block canonicalises them.
 // One block per chunk of whole rows holding at most GT_CAP off-diagonal local
 // entries (chunk: first row of each chunk, built once per operator):
-//   phase 1: GT_Q lanes per row walk a share of its terms each, over term
-//            tables staged in LDS; diagonal entries stay in registers,
-//            off-diagonal images b' and their entry ids go to LDS at the
-//            lane's offset from a block scan, so each row's targets lie in
-//            term and entry order;
-//   phase 2: lane tid takes targets tid + GT_TPB q: canonical form and lookup
-//            (position over the image, coefficient into its contribution
-//            slot), then the GT_K x gathers together;
+//   phase 1: GT_Q lanes per row walk a share of its terms each; diagonal
+//            entries stay in registers, off-diagonal images b' and their
+//            entry ids go to LDS at the lane's offset from a block scan, so
+//            each row's targets lie in term and entry order;
+//   phase 2: lane tid takes targets tid + GT_TPB q: their canonical forms one
+//            at a time, then their GT_K lookups and x gathers together;
 //   phase 3: each row's first lane adds its shares' diagonal sums, sums the
 //            row's slice in order, scales by 1/R_i and writes y.
 // A row's off-diagonal sum runs in the row kernel's order; no atomics.
@@ -191,6 +189,7 @@ static void launch_generic(const MatvecArgs &a, const VT *x, VT *y, int64_t nb, 
 #define GT_K 4
 #define GT_CAP (GT_TPB * GT_K)
 #define GT_NONE (~0ull)
+#define GT_ABSENT 0xffff  // entry-id marker: representative not in the table
 #define GT_Q 4  // lanes per row in phase 1
 
 __device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, uint32_t *wsum) {
@@ -211,7 +210,7 @@ __device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, u
   return base + xx - v;
 }
 
-template <typename VT, typename ST, bool STAGED>
+template <typename VT, typename ST>
 __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
                                                          const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate) {
@@ -219,23 +218,8 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
   __shared__ uint16_t ent[GT_CAP];   // entry id into T.col / T.val
   __shared__ VT contr[GT_CAP];       // coefficient, then contribution
   __shared__ uint32_t wsum[GT_TPB / 64];
-  __shared__ int s_mo[STAGED ? GEN_LDS_MO : 1], s_rp[STAGED ? GEN_LDS_RP : 1], s_col[STAGED ? GEN_LDS_NZ : 1];
-  __shared__ double2 s_val[STAGED ? GEN_LDS_NZ : 1];
-  TermsDev T = A.T;
+  const TermsDev &T = A.T;
   const int tid = threadIdx.x;
-  if constexpr (STAGED) {
-    for (int k = tid; k < T.nmo; k += GT_TPB) s_mo[k] = T.mrow_off[k];
-    for (int k = tid; k < T.nrp; k += GT_TPB) s_rp[k] = T.rowptr[k];
-    for (int k = tid; k < T.nnz; k += GT_TPB) {
-      s_col[k] = T.col[k];
-      s_val[k] = T.val[k];
-    }
-    __syncthreads();
-    T.mrow_off = s_mo;
-    T.rowptr = s_rp;
-    T.col = s_col;
-    T.val = s_val;
-  }
   // GT_Q lanes per row (consecutive lanes of one wave), each walking one
   // share of the terms; the block scan in lane order then lays each row's
   // targets out in term order
@@ -300,50 +284,97 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
     }
   }
   __syncthreads();
-  // phase 2a: canonical form and lookup, one target at a time; each slot is
-  // read and rewritten by its own lane only
+  // phase 2a: canonical forms, one target at a time: the coefficient into
+  // the target's contribution slot, the representative over its image, and
+  // GT_ABSENT over the entry id of an image whose representative cannot be
+  // in the table.  Each slot is read and rewritten by its own lane only.
 #pragma unroll 1
   for (int q = 0; q < GT_K; q++) {
     const uint32_t j = tid + GT_TPB * q;
     if (j < total) {
       const uint64_t t = word[j];
-      int64_t jx;
-      double cr = 1.0, ci = 0.0;
-      if (A.kind == EDK_PROJECTED) {
-        jx = edk_search(A.S, t);
-      } else {
-        const CanonOut o = edk_canon(A.C, t);
-        jx = (A.orbit_norms && !o.ok) ? -1 : edk_search(A.S, o.rep);
-        if (jx >= 0) {
-          const double Rj = A.orbit_norms ? o.R : A.R[jx];
-          cr = o.phase.x * Rj;  // conj(phase) * R_j
-          ci = -o.phase.y * Rj;
-        }
-      }
       const double2 v = T.val[ent[j]];
+      double cr = 1.0, ci = 0.0;
+      uint64_t rep = t;
+      bool ok = true;
+      if (A.kind != EDK_PROJECTED) {
+        const CanonOut o = edk_canon(A.C, t);
+        rep = o.rep;
+        ok = !(A.orbit_norms && !o.ok);
+        const double Rj = A.orbit_norms ? o.R : 1.0;  // uploaded tables: R_j after the lookup
+        cr = o.phase.x * Rj;                           // conj(phase) * R_j
+        ci = -o.phase.y * Rj;
+      }
       VT c;
       c.x = (ST)(v.x * cr - v.y * ci);
       c.y = (ST)(v.x * ci + v.y * cr);
       contr[j] = c;
-      word[j] = jx < 0 ? GT_NONE : (uint64_t)jx;
+      word[j] = rep;
+      if (!ok) ent[j] = GT_ABSENT;
     }
   }
-  // phase 2b: the lane's gathers in flight together; an absent target reads
-  // x[0] and is dropped by selection
-  uint64_t p[GT_K];
-  VT xg[GT_K];
+  // phase 2b: the lane's GT_K lookups together: directory slots, then
+  // S.nprobe halving steps (enough for the fullest slot) with the GT_K probes
+  // of a step in flight at once, then the equality check.  Inactive lookups
+  // read index 0 and are dropped by selection.
+  const SearchDev &S = A.S;
+  uint64_t key[GT_K];
+  uint32_t lo[GT_K], nn[GT_K];
+  bool act[GT_K];
 #pragma unroll
   for (int q = 0; q < GT_K; q++) {
     const uint32_t j = tid + GT_TPB * q;
-    p[q] = j < total ? word[j] : GT_NONE;
+    act[q] = j < total && ent[j] != GT_ABSENT;
+    key[q] = act[q] ? word[j] : 0;
+    const uint64_t b = key[q] >> S.dshift;
+    act[q] = act[q] && b < (uint64_t)S.ndir;
+    const uint64_t bb = act[q] ? b : 0;
+    lo[q] = S.dir[bb];
+    nn[q] = act[q] ? S.dir[bb + 1] - lo[q] : 0;
   }
+  for (int it = 0; it < S.nprobe; it++) {
+    uint64_t kv[GT_K];
 #pragma unroll
-  for (int q = 0; q < GT_K; q++) xg[q] = x[p[q] != GT_NONE ? p[q] : 0];
+    for (int q = 0; q < GT_K; q++) kv[q] = S.keys[nn[q] ? lo[q] + (nn[q] >> 1) : 0];
+#pragma unroll
+    for (int q = 0; q < GT_K; q++) {
+      const uint32_t half = nn[q] >> 1;
+      if (nn[q]) {
+        if (kv[q] < key[q]) {
+          lo[q] += half + 1;
+          nn[q] -= half + 1;
+        } else {
+          nn[q] = half;
+        }
+      }
+    }
+  }
+  uint64_t p[GT_K];
+#pragma unroll
+  for (int q = 0; q < GT_K; q++) {
+    const bool in = act[q] && (int64_t)lo[q] < S.D;
+    const uint64_t kv = S.keys[in ? lo[q] : 0];
+    p[q] = (in && kv == key[q]) ? (uint64_t)lo[q] : GT_NONE;
+  }
+  // phase 2c: the gathers in flight together (and, for uploaded tables, R_j);
+  // an absent target reads x[0] and is dropped by selection
+  VT xg[GT_K];
+  ST rj[GT_K];
+#pragma unroll
+  for (int q = 0; q < GT_K; q++) {
+    const uint64_t pq = p[q] != GT_NONE ? p[q] : 0;
+    xg[q] = x[pq];
+    rj[q] = (A.orbit_norms || A.kind == EDK_PROJECTED) ? (ST)1 : (ST)A.R[pq];
+  }
 #pragma unroll
   for (int q = 0; q < GT_K; q++) {
     const uint32_t j = tid + GT_TPB * q;
     if (j < total) {
-      const VT c = contr[j];
+      VT c = contr[j];
+      if (!A.orbit_norms && A.kind != EDK_PROJECTED) {
+        c.x *= rj[q];
+        c.y *= rj[q];
+      }
       VT cv;
       cv.x = c.x * (ST)xg[q].x - c.y * (ST)xg[q].y;
       cv.y = c.x * (ST)xg[q].y + c.y * (ST)xg[q].x;
@@ -429,7 +460,7 @@ bool edk_generic_setup(edk_op *op) {
   if (env && strcmp(env, "row") == 0) return false;
   const bool force = env && strcmp(env, "tile") == 0;
   if (!force && b->kind == EDK_PROJECTED) return false;
-  if (b->kind == EDK_TENSOR || b->D == 0 || b->D >= (1ll << 32) - 1 || op->terms.nnz > 65535) return false;
+  if (b->kind == EDK_TENSOR || b->D == 0 || b->D >= (1ll << 32) - 1 || op->terms.nnz >= GT_ABSENT) return false;
   edk_ctx *ctx = op->ctx;
   uint32_t *d_cnt = nullptr;
   std::vector<uint32_t> cnt(b->D);
@@ -478,23 +509,12 @@ static bool launch_gtile(edk_op *op, int dtype, void *y, const void *x, int accu
   }
   if (op->gstate < 0) return false;
   const MatvecArgs a = op->args();
-  const TermsDev &T = a.T;
-  const bool staged = T.nmo <= GEN_LDS_MO && T.nrp <= GEN_LDS_RP && T.nnz <= GEN_LDS_NZ;
-  if (dtype == EDK_C64) {
-    if (staged)
-      k_matvec_gtile<double2, double, true><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
-                                                                          (double2 *)y, accumulate);
-    else
-      k_matvec_gtile<double2, double, false><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
-                                                                           (double2 *)y, accumulate);
-  } else {
-    if (staged)
-      k_matvec_gtile<float2, float, true><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
-                                                                        (float2 *)y, accumulate);
-    else
-      k_matvec_gtile<float2, float, false><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
-                                                                         (float2 *)y, accumulate);
-  }
+  if (dtype == EDK_C64)
+    k_matvec_gtile<double2, double><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
+                                                                    (double2 *)y, accumulate);
+  else
+    k_matvec_gtile<float2, float><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
+                                                                 (float2 *)y, accumulate);
   return true;
 }
 
