@@ -1,5 +1,7 @@
 // edk_matvec.h -- operator tables and matvec kernel arguments.
 #pragma once
+#include <atomic>
+
 #include "edk_internal.h"
 
 #define EDK_TDESC (2 + EDK_MAXSITES)  // per term: nsites, matrix id, field shifts
@@ -42,7 +44,7 @@ struct edk_op {
   bool disable_fast = false;  // force the generic kernel (tests compare both)
   uint32_t *d_gchunk = nullptr;  // generic tile kernel: first row of each chunk
   uint32_t ngchunk = 0;
-  int gstate = 0;                 // generic tile setup: 0 not yet, 1 built, -1 not applicable
+  std::atomic<int> gstate{0};     // generic tile setup: 0 not yet, 1 built, -1 not applicable
   size_t scratch_bytes = 0;
   std::vector<int> h_tdesc, h_mrow_off, h_rowptr, h_col;
   std::vector<double2> h_val;

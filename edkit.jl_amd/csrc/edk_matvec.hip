@@ -454,6 +454,8 @@ static std::mutex g_gtile_mu;  // first-time setup may be reached from several t
 bool edk_generic_setup(edk_op *op) {
   std::lock_guard<std::mutex> lk(g_gtile_mu);
   if (op->gstate) return op->gstate > 0;
+  // the chunks are published by the final store of gstate = 1; launches that
+  // see 0 (not yet) or -1 use the row kernel
   op->gstate = -1;
   const edk_basis *b = op->basis;
   const char *env = getenv("EDK_GENERIC_SCHEDULE");
