@@ -21,7 +21,7 @@ from .linearmap import DoubleBasis, basis_embedding, symmetrizer  # noqa: F401
 from .lindblad import (DensityMatrix, Lindblad, LindbladArnoldiCache, LiouvillianMap,  # noqa: F401
                        densitymatrix, lindblad, lindblad_timeevolve, lindblad_timeevolve_)
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
-from .matvec import (Array, apply, clear_sparse_cache_, device_operator, matvec_into, mul,  # noqa: F401
+from .matvec import (Array, addto_, apply, clear_sparse_cache_, device_operator, matvec_into, mul,  # noqa: F401
                      mul_, sparse, sparse_)
 from .operator import Operator, find_base, operator, trans_inv_operator  # noqa: F401
 from .serve import Server  # noqa: F401

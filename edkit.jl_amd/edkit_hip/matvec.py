@@ -157,6 +157,30 @@ def Array(H):
     return sparse(H).toarray()
 
 
+def addto_(M, H):
+    """addto!(M, opt) (Operator.jl:171-177): accumulate the sector matrix into the
+    preallocated M and return M.  A numpy array is updated in place.  A scipy
+    CSR or CSC matrix takes over the storage of M + sparse(opt), so the object
+    the caller holds carries the sum, as the reference's SparseMatrixCSC does.
+    The shapes must agree, and M's element type must hold the matrix's."""
+    import scipy.sparse as sp
+    if tuple(M.shape) != tuple(H.shape):
+        raise ValueError(f"addto_: matrix of shape {tuple(M.shape)} for an operator of shape {tuple(H.shape)}")
+    if sp.issparse(M):
+        if M.format not in ("csr", "csc"):
+            raise TypeError("addto_: a sparse target must be CSR or CSC")
+        S = sparse(H)
+        if not np.can_cast(S.dtype, M.dtype, casting="same_kind"):
+            raise TypeError(f"addto_: cannot accumulate {S.dtype} entries into a {M.dtype} matrix")
+        T = (M + S).asformat(M.format)
+        T.sum_duplicates()
+        M.data, M.indices, M.indptr = T.data.astype(M.dtype, copy=False), T.indices, T.indptr
+        M.has_sorted_indices = True
+        return M
+    M += Array(H)
+    return M
+
+
 # sparse! keeps at most 8 assembled operators, least recently used evicted
 # (the reference's LRU{UInt, Any}(maxsize=8), Operator.jl:221)
 _SPARSE_CACHE = collections.OrderedDict()

@@ -1,5 +1,5 @@
-"""GPU parity of the explicit sector matrix (sparse(opt), Array(opt), the
-sparse!(opt) cache, Operator.jl:171-285): the device-assembled matrix against
+"""GPU parity of the explicit sector matrix (sparse(opt), Array(opt), addto!,
+the sparse!(opt) cache, Operator.jl:171-285): the device-assembled matrix against
 the oracle's sector matrix column by column (the oracle's scatter mul on unit
 vectors), within 1e-12; products with several right-hand sides through the
 cached matrix against the matrix-free kernels."""
@@ -77,3 +77,29 @@ def test_cached_matrix_serves_block_products(gpu):
     assert np.linalg.norm((H @ Xd).to_host() - y_free) <= 1e-12 * np.linalg.norm(y_free)
     # Hermitian sector matrix
     assert abs(S - S.conj().T).max() <= 1e-12
+
+
+def test_addto_accumulates_in_place(gpu):
+    """addto!(M, opt) adds the sector matrix into M and returns M: a dense array
+    in place, a CSR and a CSC matrix through their storage; shape and element
+    type mismatches raise."""
+    import scipy.sparse as sp
+    import edkit_hip as edk
+    B, ob = pair(edk, "TranslationalBasis", 12, 2, dict(N=6, k=5))
+    H = j1j2(B, 0.3)
+    op = orc.Op(ob, H.flatten())
+    ref = np.stack([op @ np.eye(B.D, dtype=np.complex128)[:, j] for j in range(B.D)], axis=1)
+    rng = np.random.default_rng(8)
+    M0 = rng.standard_normal((B.D, B.D)) + 1j * rng.standard_normal((B.D, B.D))
+    M = M0.copy()
+    assert edk.addto_(M, H) is M
+    assert np.abs(M - (M0 + ref)).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    for fmt in ("csr", "csc"):
+        S0 = sp.random(B.D, B.D, density=0.05, random_state=3, format=fmt) * (1 + 0.5j)
+        S = S0.copy()
+        assert edk.addto_(S, H) is S and S.format == fmt
+        assert np.abs(S.toarray() - (S0.toarray() + ref)).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    with pytest.raises(ValueError):
+        edk.addto_(np.zeros((B.D + 1, B.D), complex), H)
+    with pytest.raises(TypeError):
+        edk.addto_(sp.csr_matrix((B.D, B.D), dtype=np.float64), H)
