@@ -15,7 +15,8 @@ from .batch import MatvecBatch  # noqa: F401
 from .schmidt import ent_S, ent_spec, entropy, schmidt  # noqa: F401
 from .basis import (AbelianBasis, AbstractBasis, FlipBasis, ParityBasis,  # noqa: F401
                     ParityFlipBasis, ProjectedBasis, TensorBasis, TranslationalBasis,
-                    TranslationFlipBasis, TranslationParityBasis, basis, productstate)
+                    TranslationFlipBasis, TranslationParityBasis, basis, change_, content, index, norm,
+                    productstate)
 from .device import Context, DeviceArray, default_context, set_default_context  # noqa: F401
 from .linearmap import DoubleBasis, basis_embedding, symmetrizer  # noqa: F401
 from .lindblad import (DensityMatrix, Lindblad, LindbladArnoldiCache, LiouvillianMap,  # noqa: F401

@@ -28,6 +28,7 @@ class AbstractBasis:
     def __init__(self, kind, L, base=2, a=1, N=None, k=0, p=1, z=0, has_k=False,
                  has_p=False, has_z=False, symmetries=None, f=None, ctx=None):
         self.ctx = ctx or default_context()
+        self.kind = int(kind)
         self.L, self.base, self.a = int(L), int(base), int(a)
         self.N, self.k, self.p, self.z = N, k, p, z
         spec = BasisSpec(kind, self.L, self.base, self.a, -1 if N is None else int(N),
@@ -210,6 +211,61 @@ def basis(L, N=None, k=None, a=1, p=None, z=None, symmetries=None, base=2, f=Non
                            has_k=k is not None, has_p=p is not None, has_z=z is not None,
                            symmetries=symmetries, f=f, ctx=ctx)
     return b
+
+
+def _host_tables(B):
+    """b.I and b.R downloaded once per basis (the tables do not change)."""
+    t = getattr(B, "_host_tab", None)
+    if t is None:
+        t = B.tables()
+        B._host_tab = t
+    return t
+
+
+def _check_i(B, i):
+    i = int(i)
+    if not 1 <= i <= B.D:
+        raise IndexError(f"basis index {i} outside 1..{B.D}")
+    return i
+
+
+def content(B, i):
+    """content(b, i) (AbstractBasis.jl:95, :240): the stored label b.I[i] of the
+    i-th (1-based) basis state, the representative's integer content plus one;
+    i itself for TensorBasis."""
+    i = _check_i(B, i)
+    if B.kind == _lib.EDK_TENSOR:
+        return i
+    return int(_host_tables(B)[0][i - 1])
+
+
+def norm(B, i):
+    """norm(b, i) (AbstractBasis.jl:106-107, :241): b.R[i]; 1 for TensorBasis
+    and ProjectedBasis."""
+    i = _check_i(B, i)
+    if B.kind in (_lib.EDK_TENSOR, _lib.EDK_PROJECTED):
+        return 1.0
+    return float(_host_tables(B)[1][i - 1])
+
+
+def change_(B, i):
+    """change!(b, i) (AbstractBasis.jl:160-163): the digits of the i-th basis
+    state (site 1 first) and its norm.  The reference writes the digits into
+    b.dgt and returns the norm; here both are returned."""
+    return B.digits(content(B, i) - 1), norm(B, i)
+
+
+def index(B, dgt):
+    """index(b, dgt) (ProjectedBasis.jl:52-57, TranslationalBasis.jl:310-341,
+    TranslationalParityBasis.jl:192-234, AbelianBasis.jl:669-693, ...): the
+    coefficient and 1-based position of the digit row dgt (site 1 first) in B;
+    (0, 1) when its representative is not in the sector."""
+    dgt = np.asarray(dgt)
+    if dgt.shape != (B.L,) or np.any(dgt < 0) or np.any(dgt >= B.base):
+        raise ValueError(f"index: expected {B.L} digits in 0..{B.base - 1}")
+    coef, pos = B.index([B.content(dgt)])
+    c = complex(coef[0])
+    return (c if c.imag != 0 else c.real), int(pos[0])
 
 
 def productstate(v, B):
