@@ -206,6 +206,69 @@ __global__ void __launch_bounds__(RED_TPB) k_gemv_t(const VT *V, int64_t ld_v, i
   }
 }
 
+// k_gemv_t for a ComplexF32 basis: two rows per lane with 16-byte loads (one
+// 8-byte load per lane per column leaves the kernel far below the copy rate),
+// FP64 accumulation.  Needs an even leading dimension and 16-byte aligned
+// V and w; an odd last row is added by lane 0 of block 0.
+__global__ void __launch_bounds__(RED_TPB) k_gemv_t_c32x2(const float2 *V, int64_t ld_v, int j, const float2 *w,
+                                                          int64_t n, double2 *part) {
+  __shared__ double2 sm[RED_TPB / 64][GEMV_CG];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t n2 = n >> 1;
+  const bool tail = (n & 1) && blockIdx.x == 0 && threadIdx.x == 0;
+  for (int c0 = 0; c0 < j; c0 += GEMV_CG) {
+    const int cg = min(GEMV_CG, j - c0);
+    double2 acc[GEMV_CG];
+#pragma unroll
+    for (int q = 0; q < GEMV_CG; q++) acc[q] = make_double2(0, 0);
+    for (int64_t p = (int64_t)blockIdx.x * RED_TPB + threadIdx.x; p < n2; p += (int64_t)gridDim.x * RED_TPB) {
+      const float4 y = reinterpret_cast<const float4 *>(w)[p];
+#pragma unroll
+      for (int q = 0; q < GEMV_CG; q++) {
+        if (q < cg) {
+          const float4 x = reinterpret_cast<const float4 *>(V + (int64_t)(c0 + q) * ld_v)[p];
+          acc[q].x += (double)x.x * y.x + (double)x.y * y.y;
+          acc[q].y += (double)x.x * y.y - (double)x.y * y.x;
+          acc[q].x += (double)x.z * y.z + (double)x.w * y.w;
+          acc[q].y += (double)x.z * y.w - (double)x.w * y.z;
+        }
+      }
+    }
+    if (tail) {
+      const float2 y = w[n - 1];
+#pragma unroll
+      for (int q = 0; q < GEMV_CG; q++) {
+        if (q < cg) {
+          const float2 x = V[(int64_t)(c0 + q) * ld_v + n - 1];
+          acc[q].x += (double)x.x * y.x + (double)x.y * y.y;
+          acc[q].y += (double)x.x * y.y - (double)x.y * y.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < GEMV_CG; q++) {
+      for (int o = 32; o > 0; o >>= 1) {
+        acc[q].x += __shfl_down(acc[q].x, o, 64);
+        acc[q].y += __shfl_down(acc[q].y, o, 64);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < GEMV_CG; q++) sm[wid][q] = acc[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < cg) {
+      double2 r = make_double2(0, 0);
+      for (int u = 0; u < RED_TPB / 64; u++) {
+        r.x += sm[u][threadIdx.x].x;
+        r.y += sm[u][threadIdx.x].y;
+      }
+      part[(int64_t)(c0 + threadIdx.x) * gridDim.x + blockIdx.x] = r;
+    }
+  }
+}
+
 // out = scale_in * out_in - sum_c V[:, c] s[c]   (mode 0: w -= V s)
 // out = scale * sum_c V[:, c] s[c]               (mode 1: reconstruct)
 template <typename VT, typename VO>
@@ -361,7 +424,10 @@ int KrylovWork::scale(void *y, const void *x, double s) {
 // One classical Gram-Schmidt sweep: h = V[:, :j]^H w, w -= V[:, :j] h.
 int KrylovWork::cgs_sweep(const void *V, int64_t ldv, int j, void *w) {
   hipStream_t s = ctx->stream;
-  DISPATCH(VT, k_gemv_t<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)V, ldv, j, (const VT *)w, n, d_part));
+  if (dtype == EDK_C32 && ldv % 2 == 0 && ((uintptr_t)V & 15) == 0 && ((uintptr_t)w & 15) == 0)
+    k_gemv_t_c32x2<<<nblk, RED_TPB, 0, s>>>((const float2 *)V, ldv, j, (const float2 *)w, n, d_part);
+  else
+    DISPATCH(VT, k_gemv_t<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)V, ldv, j, (const VT *)w, n, d_part));
   k_reduce_part<<<j, RED_TPB, 0, s>>>(d_part, nblk, d_s);
   unsigned g = (unsigned)((n + RED_TPB - 1) / RED_TPB);
   DISPATCH(VT, k_gemv_n<VT, VT><<<g, RED_TPB, 0, s>>>((const VT *)V, ldv, j, d_s, (VT *)w, n, 0, 0.0));
