@@ -442,7 +442,7 @@ static std::mutex g_gtile_mu;  // first-time setup may be reached from several t
 
 // Chunks of whole rows with at most GT_CAP off-diagonal entries and
 // GT_TPB / GT_Q rows each.  Not applicable: tensor bases (no table), more than
-// 2^32 - 2 rows, entry ids beyond 16 bits, a row with more than GT_CAP
+// 2^31 rows, entry ids beyond 16 bits, a row with more than GT_CAP
 // entries.  Used by default for every basis that needs a canonical form, where
 // it beat the row kernel at every size measured (ComplexF64, row -> tile):
 // 5 x 5 torus at k = (1, 2), D = 208 012: 0.88 -> 0.39 ms; 6 x 5 torus,
@@ -462,7 +462,8 @@ bool edk_generic_setup(edk_op *op) {
   if (env && strcmp(env, "row") == 0) return false;
   const bool force = env && strcmp(env, "tile") == 0;
   if (!force && b->kind == EDK_PROJECTED) return false;
-  if (b->kind == EDK_TENSOR || b->D == 0 || b->D >= (1ll << 32) - 1 || op->terms.nnz >= GT_ABSENT) return false;
+  // (the per-row counts are copied to the host: at most 2^31 rows, 8 GB)
+  if (b->kind == EDK_TENSOR || b->D == 0 || b->D > (1ll << 31) || op->terms.nnz >= GT_ABSENT) return false;
   edk_ctx *ctx = op->ctx;
   uint32_t *d_cnt = nullptr;
   std::vector<uint32_t> cnt(b->D);

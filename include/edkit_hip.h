@@ -5,7 +5,7 @@
  * EDKit.jl has no FFI layer of its own: its boundary is Julia multiple
  * dispatch.  Each entry point below names the Julia method it stands in for
  * (file:line in the EDKit.jl v0.5.0 tree); a Julia binding calls these through
- * `ccall` (the binding is given in INTEGRATION.md next to this header).
+ * `ccall`, with plain pointers and sizes as declared here.
  *
  * Conventions
  *   - Every function returns an int status (EDK_OK == 0); on failure the
