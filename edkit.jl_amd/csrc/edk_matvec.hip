@@ -214,8 +214,8 @@ template <typename VT, typename ST>
 __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
                                                          const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate) {
-  __shared__ uint64_t word[GT_CAP];  // image b', then its position (GT_NONE: absent)
-  __shared__ uint16_t ent[GT_CAP];   // entry id into T.col / T.val
+  __shared__ uint64_t word[GT_CAP];  // image b', then its representative
+  __shared__ uint16_t ent[GT_CAP];   // entry id into T.col / T.val, or GT_ABSENT
   __shared__ VT contr[GT_CAP];       // coefficient, then contribution
   __shared__ uint32_t wsum[GT_TPB / 64];
   const TermsDev &T = A.T;
