@@ -499,7 +499,9 @@ int KrylovWork::cgs_sweep(const void *V, int64_t ldv, int j, void *w) {
   hipStream_t s = ctx->stream;
   if (dtype == EDK_C32 && ldv % 2 == 0 && ((uintptr_t)V & 15) == 0 && ((uintptr_t)w & 15) == 0)
     k_gemv_t_c32x2<<<nblk, RED_TPB, 0, s>>>((const float2 *)V, ldv, j, (const float2 *)w, n, d_part);
-  else if (dtype == EDK_C64)
+  else if (dtype == EDK_C64 && n >= (int64_t)RED_TPB * 8 * nblk)
+    // every block has at least one tile (about 2.1 M rows and up); below
+    // that most blocks would idle (L = 24 evolution: 2120 -> 2000 outputs/s)
     k_gemv_t_tile<8><<<nblk, RED_TPB, 0, s>>>((const double2 *)V, ldv, j, (const double2 *)w, n, d_part);
   else
     DISPATCH(VT, k_gemv_t<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)V, ldv, j, (const VT *)w, n, d_part));
