@@ -210,11 +210,17 @@ __device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, u
   return base + xx - v;
 }
 
-template <typename VT, typename ST>
+// NV = 0: one vector (x, y plain).  NV >= 1: columns [col, col + NV) of a
+// row-interleaved block of nb columns; phase 2 keeps each target's position and
+// coefficient in LDS and phase 3 gives each row NV lanes, one per column, that
+// walk the row's targets in order (each lookup serves all NV columns).
+template <typename VT, typename ST, int NV>
 __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
                                                          const VT *__restrict__ x, VT *__restrict__ y,
-                                                         int accumulate) {
-  __shared__ uint64_t word[GT_CAP];  // image b', then its representative
+                                                         int accumulate, int64_t nb, int64_t col) {
+  __shared__ uint32_t s_off[NV ? GT_TPB / GT_Q : 1], s_n[NV ? GT_TPB / GT_Q : 1];
+  __shared__ VT s_dg[NV ? GT_TPB / GT_Q : 1];
+  __shared__ uint64_t word[GT_CAP];  // image b', then its representative (blocks: its position)
   __shared__ uint16_t ent[GT_CAP];   // entry id into T.col / T.val, or GT_ABSENT
   __shared__ VT contr[GT_CAP];       // coefficient, then contribution
   __shared__ uint32_t wsum[GT_TPB / 64];
@@ -356,35 +362,7 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
     const uint64_t kv = S.keys[in ? lo[q] : 0];
     p[q] = (in && kv == key[q]) ? (uint64_t)lo[q] : GT_NONE;
   }
-  // phase 2c: the gathers in flight together (and, for uploaded tables, R_j);
-  // an absent target reads x[0] and is dropped by selection
-  VT xg[GT_K];
-  ST rj[GT_K];
-#pragma unroll
-  for (int q = 0; q < GT_K; q++) {
-    const uint64_t pq = p[q] != GT_NONE ? p[q] : 0;
-    xg[q] = x[pq];
-    rj[q] = (A.orbit_norms || A.kind == EDK_PROJECTED) ? (ST)1 : (ST)A.R[pq];
-  }
-#pragma unroll
-  for (int q = 0; q < GT_K; q++) {
-    const uint32_t j = tid + GT_TPB * q;
-    if (j < total) {
-      VT c = contr[j];
-      if (!A.orbit_norms && A.kind != EDK_PROJECTED) {
-        c.x *= rj[q];
-        c.y *= rj[q];
-      }
-      VT cv;
-      cv.x = c.x * (ST)xg[q].x - c.y * (ST)xg[q].y;
-      cv.y = c.x * (ST)xg[q].y + c.y * (ST)xg[q].x;
-      if (p[q] == GT_NONE) cv.x = cv.y = 0;
-      contr[j] = cv;
-    }
-  }
-  __syncthreads();
-  // phase 3: the first lane of each row gathers its shares' counts and
-  // diagonal sums (share order), then sums the row's slice in order
+  // the row's shares: counts and diagonal sums (share order), at its first lane
   const int lead = (tid & 63) - qtr;  // the row's first lane within the wave
   uint32_t nr = 0;
   ST dsr = 0, dsi = 0;
@@ -394,28 +372,116 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
     dsr += __shfl(dr, lead + q, 64);
     dsi += __shfl(di, lead + q, 64);
   }
-  if (own && qtr == 0) {
-    const int64_t i = (int64_t)r0 + row;
-    const VT xi = x[i];
-    const ST invR = A.kind != EDK_PROJECTED ? (ST)(1.0 / A.R[i]) : (ST)1;
-    ST orr = 0, oi = 0;
-    for (uint32_t q = off; q < off + nr; q++) {
-      const VT cv = contr[q];
-      orr += cv.x;
-      oi += cv.y;
+  if constexpr (NV == 0) {
+    // phase 2c: the gathers in flight together (and, for uploaded tables,
+    // R_j); an absent target reads x[0] and is dropped by selection
+    VT xg[GT_K];
+    ST rj[GT_K];
+#pragma unroll
+    for (int q = 0; q < GT_K; q++) {
+      const uint64_t pq = p[q] != GT_NONE ? p[q] : 0;
+      xg[q] = x[pq];
+      rj[q] = (A.orbit_norms || A.kind == EDK_PROJECTED) ? (ST)1 : (ST)A.R[pq];
     }
-    const ST yr = dsr * (ST)xi.x - dsi * (ST)xi.y + orr * invR;
-    const ST yi = dsr * (ST)xi.y + dsi * (ST)xi.x + oi * invR;
-    VT out;
-    if (accumulate) {
-      const VT y0 = y[i];
-      out.x = y0.x + yr;
-      out.y = y0.y + yi;
-    } else {
-      out.x = yr;
-      out.y = yi;
+#pragma unroll
+    for (int q = 0; q < GT_K; q++) {
+      const uint32_t j = tid + GT_TPB * q;
+      if (j < total) {
+        VT c = contr[j];
+        if (!A.orbit_norms && A.kind != EDK_PROJECTED) {
+          c.x *= rj[q];
+          c.y *= rj[q];
+        }
+        VT cv;
+        cv.x = c.x * (ST)xg[q].x - c.y * (ST)xg[q].y;
+        cv.y = c.x * (ST)xg[q].y + c.y * (ST)xg[q].x;
+        if (p[q] == GT_NONE) cv.x = cv.y = 0;
+        contr[j] = cv;
+      }
     }
-    y[i] = out;
+    __syncthreads();
+    // phase 3: each row's first lane sums its slice in order
+    if (own && qtr == 0) {
+      const int64_t i = (int64_t)r0 + row;
+      const VT xi = x[i];
+      const ST invR = A.kind != EDK_PROJECTED ? (ST)(1.0 / A.R[i]) : (ST)1;
+      ST orr = 0, oi = 0;
+      for (uint32_t q = off; q < off + nr; q++) {
+        const VT cv = contr[q];
+        orr += cv.x;
+        oi += cv.y;
+      }
+      const ST yr = dsr * (ST)xi.x - dsi * (ST)xi.y + orr * invR;
+      const ST yi = dsr * (ST)xi.y + dsi * (ST)xi.x + oi * invR;
+      VT out;
+      if (accumulate) {
+        const VT y0 = y[i];
+        out.x = y0.x + yr;
+        out.y = y0.y + yi;
+      } else {
+        out.x = yr;
+        out.y = yi;
+      }
+      y[i] = out;
+    }
+  } else {
+    // phase 2c: positions over the representatives (own slots); uploaded
+    // tables scale the coefficient by R_j now
+#pragma unroll
+    for (int q = 0; q < GT_K; q++) {
+      const uint32_t j = tid + GT_TPB * q;
+      if (j < total) {
+        word[j] = p[q];
+        if (!A.orbit_norms && A.kind != EDK_PROJECTED && p[q] != GT_NONE) {
+          const ST r = (ST)A.R[p[q]];
+          VT c = contr[j];
+          c.x *= r;
+          c.y *= r;
+          contr[j] = c;
+        }
+      }
+    }
+    if (own && qtr == 0) {
+      s_off[row] = off;
+      s_n[row] = nr;
+      VT d;
+      d.x = dsr;
+      d.y = dsi;
+      s_dg[row] = d;
+    }
+    __syncthreads();
+    // phase 3: NV lanes per row, one per column, each walking the row's
+    // targets in order; the NV values of a target are contiguous in x
+    for (uint32_t idx = tid; idx < nrow * NV; idx += GT_TPB) {
+      const uint32_t rw = idx / NV, c = idx % NV;
+      const int64_t i = (int64_t)r0 + rw;
+      const uint32_t o = s_off[rw], cnt = s_n[rw];
+      ST orr = 0, oi = 0;
+      for (uint32_t q = o; q < o + cnt; q++) {
+        const uint64_t pq = word[q];
+        if (pq != GT_NONE) {
+          const VT cf = contr[q];
+          const VT xv = x[(int64_t)pq * nb + col + c];
+          orr += cf.x * (ST)xv.x - cf.y * (ST)xv.y;
+          oi += cf.x * (ST)xv.y + cf.y * (ST)xv.x;
+        }
+      }
+      const VT dg = s_dg[rw];
+      const VT xi = x[i * nb + col + c];
+      const ST invR = A.kind != EDK_PROJECTED ? (ST)(1.0 / A.R[i]) : (ST)1;
+      const ST yr = dg.x * (ST)xi.x - dg.y * (ST)xi.y + orr * invR;
+      const ST yi = dg.x * (ST)xi.y + dg.y * (ST)xi.x + oi * invR;
+      VT out;
+      if (accumulate) {
+        const VT y0 = y[i * nb + col + c];
+        out.x = y0.x + yr;
+        out.y = y0.y + yi;
+      } else {
+        out.x = yr;
+        out.y = yi;
+      }
+      y[i * nb + col + c] = out;
+    }
   }
 }
 
@@ -503,8 +569,34 @@ bool edk_generic_setup(edk_op *op) {
   return true;
 }
 
+template <typename VT, typename ST>
+static void launch_gtile_t(const edk_op *op, const MatvecArgs &a, const VT *x, VT *y, int64_t nb, int accumulate,
+                           hipStream_t st) {
+  if (nb == 1) {
+    k_matvec_gtile<VT, ST, 0><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, x, y, accumulate, 1, 0);
+    return;
+  }
+  for (int64_t c = 0; c < nb;) {
+    const int64_t left = nb - c;
+    if (left >= 8) {
+      k_matvec_gtile<VT, ST, 8><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, x, y, accumulate, nb, c);
+      c += 8;
+    } else if (left >= 4) {
+      k_matvec_gtile<VT, ST, 4><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, x, y, accumulate, nb, c);
+      c += 4;
+    } else if (left >= 2) {
+      k_matvec_gtile<VT, ST, 2><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, x, y, accumulate, nb, c);
+      c += 2;
+    } else {
+      k_matvec_gtile<VT, ST, 1><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, x, y, accumulate, nb, c);
+      c += 1;
+    }
+  }
+}
+
 // the tile form when it is set up (or can be: never under stream capture)
-static bool launch_gtile(edk_op *op, int dtype, void *y, const void *x, int accumulate, hipStream_t st) {
+static bool launch_gtile(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate,
+                         hipStream_t st) {
   if (op->gstate == 0) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
@@ -513,11 +605,9 @@ static bool launch_gtile(edk_op *op, int dtype, void *y, const void *x, int accu
   if (op->gstate < 0) return false;
   const MatvecArgs a = op->args();
   if (dtype == EDK_C64)
-    k_matvec_gtile<double2, double><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const double2 *)x,
-                                                                    (double2 *)y, accumulate);
+    launch_gtile_t<double2, double>(op, a, (const double2 *)x, (double2 *)y, nb, accumulate, st);
   else
-    k_matvec_gtile<float2, float><<<op->ngchunk, GT_TPB, 0, st>>>(a, op->d_gchunk, (const float2 *)x,
-                                                                 (float2 *)y, accumulate);
+    launch_gtile_t<float2, float>(op, a, (const float2 *)x, (float2 *)y, nb, accumulate, st);
   return true;
 }
 
@@ -744,7 +834,7 @@ int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
   // several right-hand sides with the assembled matrix cached (sparse!)
   if (nb > 1 && edk_matvec_csr(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
-  if (nb == 1 && launch_gtile(op, dtype, y, x, accumulate, st)) {
+  if (launch_gtile(op, dtype, y, x, nb, accumulate, st)) {
     EDK_HIP(hipGetLastError());
     return EDK_OK;
   }

@@ -2,7 +2,8 @@
 form with the targets compacted into LDS) against the oracle's scatter mul
 (Operator.jl:399-465) on the same seeded inputs, for every basis kind that
 reaches them: ComplexF64 to 1e-12, the accumulating mul! form, ComplexF32 to
-2e-6, and the two schedules against each other.  EDK_GENERIC_SCHEDULE picks
+2e-6, the two schedules against each other, and blocks of vectors.
+EDK_GENERIC_SCHEDULE picks
 the schedule when an operator is first used, so each schedule gets an
 operator of its own."""
 import numpy as np
@@ -61,3 +62,29 @@ def test_generic_schedules_match_oracle(gpu, monkeypatch, kind, L, base, kw, mod
         assert np.linalg.norm(y32 - yo32) <= 2e-6 * np.linalg.norm(yo32), sched
         out[sched] = y
     assert np.abs(out["row"] - out["tile"]).max() <= 1e-13 * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("kind,L,base,kw,model", CASES)
+def test_generic_schedules_blocks(gpu, monkeypatch, kind, L, base, kw, model):
+    """Blocks of 3 (a 2-column group and a single column) and 8 row-interleaved
+    vectors through both schedules: every column against the oracle's mul on
+    that column, ComplexF64 and ComplexF32, and the accumulating form."""
+    import edkit_hip as edk
+    B, ob = _pair(edk, kind, L, base, kw)
+    build = (lambda: heisenberg_2d(B, 4, 4, delta=0.8)) if kind == "torus" else (lambda: model(B))
+    op = orc.Op(ob, build().flatten())
+    for nbk in (3, 8):
+        X = np.stack([rand_state(B.D, 40 + c) for c in range(nbk)], axis=1)
+        ref = np.stack([op @ X[:, c] for c in range(nbk)], axis=1)
+        ref32 = np.stack([op @ X[:, c].astype(np.complex64).astype(np.complex128) for c in range(nbk)], axis=1)
+        scale = np.linalg.norm(ref)
+        for sched in ("row", "tile"):
+            monkeypatch.setenv("EDK_GENERIC_SCHEDULE", sched)
+            H = build()
+            Y = (H @ edk.DeviceArray.from_host(X)).to_host()
+            assert np.linalg.norm(Y - ref) <= 1e-12 * scale, (sched, nbk)
+            Yacc = edk.DeviceArray.from_host(X)
+            edk.mul_(Yacc, H, edk.DeviceArray.from_host(X))
+            assert np.linalg.norm(Yacc.to_host() - (X + ref)) <= 1e-12 * np.linalg.norm(X + ref), (sched, nbk)
+            Y32 = (H @ edk.DeviceArray.from_host(X.astype(np.complex64))).to_host()
+            assert np.linalg.norm(Y32 - ref32) <= 2e-6 * np.linalg.norm(ref32), (sched, nbk)
