@@ -389,3 +389,24 @@ def _oracle_ground(L, base, k, c, H):
         op = orc.Op(ob, H.flatten())
         _ORACLE_E0[key] = orc.lanczos_ground(op, rand_state(ob.D, 17), 300, 0.0, nthreads=8)[0]
     return _ORACLE_E0[key]
+
+
+def test_lanczos_large_sector_reorth_kernels(gpu):
+    """L = 30 XXZ ring (Delta = 1), TranslationParityBasis N = 15, k = 0, p = +1:
+    D = 2 587 018 rows, above the 2^21 from which the ComplexF64
+    reorthogonalisation reads w once per 2048-row tile.  Converged Lanczos runs
+    through three different reorthogonalisation paths agree on E0: ComplexF64
+    with full reorthogonalisation (the tile GEMV), ComplexF32 vectors with FP64
+    refinement (the two-rows-per-lane GEMV), and partial reorthogonalisation.
+    (The sector's lowest level is not the chain's ground state: for L/2 odd
+    that lies at k = pi.)"""
+    import edkit_hip as edk
+    B = edk.TranslationParityBasis(L=30, N=15, k=0, p=1)
+    assert B.D == 2587018
+    H = xxz(B)
+    x0 = rand_state(B.D, 5)
+    E64, r64, s64, _ = edk.lanczos_ground(H, x0, m=300, tol=1e-10)
+    E32, _, _, _ = edk.lanczos_ground(H, x0, m=300, tol=1e-10, dtype=np.complex64)
+    Ep, rp, _, _ = edk.lanczos_ground(H, x0, m=300, tol=1e-10, reorth="partial")
+    assert r64 <= 1e-10 and rp <= 1e-10 and s64 < 300
+    assert abs(E64 - E32) < 1e-9 and abs(E64 - Ep) < 1e-11
