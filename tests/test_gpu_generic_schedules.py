@@ -88,3 +88,26 @@ def test_generic_schedules_blocks(gpu, monkeypatch, kind, L, base, kw, model):
             assert np.linalg.norm(Yacc.to_host() - (X + ref)) <= 1e-12 * np.linalg.norm(X + ref), (sched, nbk)
             Y32 = (H @ edk.DeviceArray.from_host(X.astype(np.complex64))).to_host()
             assert np.linalg.norm(Y32 - ref32) <= 2e-6 * np.linalg.norm(ref32), (sched, nbk)
+
+
+@pytest.mark.parametrize("L,base,kw,model", [(12, 2, dict(N=6, k=3), j1j2), (8, 3, dict(N=8, k=2), blbq)])
+def test_uploaded_tables_through_tile_kernel(gpu, L, base, kw, model):
+    """A predicate that keeps every state makes the basis upload its tables, so
+    the norms R_j are gathered from the table instead of recomputed from the
+    orbit (and the specialised kernels do not apply).  One vector and a block of
+    4 through the tile kernel against the oracle on the unfiltered sector."""
+    import edkit_hip as edk
+    B = edk.TranslationalBasis(L=L, base=base, f=lambda d: True, **kw)
+    ob = orc.Basis("translational", L, base=base, **kw)
+    oI, oR = ob.tables()
+    I, R = B.tables()
+    assert np.array_equal(I, oI) and np.array_equal(R, oR)
+    H = model(B)
+    op = orc.Op(ob, H.flatten())
+    x = rand_state(B.D, 51)
+    yo = op @ x
+    assert np.linalg.norm(H @ x - yo) <= 1e-12 * np.linalg.norm(yo)
+    X = np.stack([rand_state(B.D, 52 + c) for c in range(4)], axis=1)
+    ref = np.stack([op @ X[:, c] for c in range(4)], axis=1)
+    Y = (H @ edk.DeviceArray.from_host(X)).to_host()
+    assert np.linalg.norm(Y - ref) <= 1e-12 * np.linalg.norm(ref)
