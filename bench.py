@@ -36,10 +36,10 @@ L, NUP, DELTA = 32, 16, 1.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "matvec_bal_L32_tp_k0.md")
 
 
-def profiled_traffic():
-    """HBM bytes per launch recorded in the committed rocprofv3 summary, or None."""
+def profiled_traffic(path=TRAFFIC_PROFILE):
+    """HBM bytes per launch recorded in a committed rocprofv3 summary, or None."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch ="):
                     return int(line.split("=")[1])
@@ -694,7 +694,9 @@ def c7_workload(args, ev):
                                "(T_y, 0)]), ComplexF64 H*psi", "D": D, "nnz_offdiag": nnz_off,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_alg_per_launch": b_alg,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(os.path.join(ROOT, "profiles", "matvec_generic_torus6x5_custom.md")),
+                     "bytes_alg_per_launch": b_alg,
                      "kernel_ms": kern_ms},
         "basis_build_s": t_build,
     }
