@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(RED_TPB) k_gemv_n(const VT *V, int64_t ld_v, i
 // host helpers
 // ---------------------------------------------------------------------------
 static int red_blocks(int64_t n) {
-  int64_t b = (n + RED_TPB * 4 - 1) / (RED_TPB * 4);
+  int64_t b = (n + RED_TPB - 1) / RED_TPB;
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, 1024));
 }
 
