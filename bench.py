@@ -36,16 +36,35 @@ L, NUP, DELTA = 32, 16, 1.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "matvec_bal_L32_tp_k0.md")
 
 
-def profiled_traffic(path=TRAFFIC_PROFILE):
-    """HBM bytes per launch recorded in a committed rocprofv3 summary, or None."""
+# the sources the default matvec kernel is compiled from
+KERNEL_SOURCES = [os.path.join(ROOT, "edkit.jl_amd", "csrc", f)
+                  for f in ("edk_fast.hip", "edk_fast.h", "edk_internal.h")]
+
+
+def kernel_source_hash(paths=KERNEL_SOURCES):
+    """sha256 over the kernel's source files (the key a profile summary records)."""
+    import hashlib
+    h = hashlib.sha256()
+    for p in paths:
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def profiled_traffic(path=TRAFFIC_PROFILE, sources=KERNEL_SOURCES):
+    """HBM bytes per launch recorded in a committed rocprofv3 summary, or None
+    when the summary was taken on other kernel sources than the ones built."""
+    traffic = key = None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch ="):
-                    return int(line.split("=")[1])
+                    traffic = int(line.split("=")[1])
+                elif line.startswith("kernel_sources_sha256 ="):
+                    key = line.split("=")[1].strip()
     except OSError:
-        pass
-    return None
+        return None
+    return traffic if key is not None and key == kernel_source_hash(sources) else None
 
 
 class HipEvents:
@@ -93,10 +112,20 @@ def time_matvec(edk, H, x, y, steps, warmup, ev, stream):
     return ev.elapsed_ms(e0, e1) / steps, wall * 1e3 / steps
 
 
+def host_threads():
+    """Threads for the CPU legs: every core this process may run on, at most
+    16 (a one-GPU box grants its jobs 16 cores whatever nproc reports)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(H, x, budget_s=12.0):
     """Oracle (C restatement of EDKit's threaded mul) on a strided column sample."""
     from oracle import oracle as orc
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads = host_threads()
     t0 = time.perf_counter()
     ob = orc.Basis("translation_parity", L, N=NUP, k=0, p=1, nthreads=nthreads)
     t_basis = time.perf_counter() - t0
@@ -124,6 +153,7 @@ def cpu_baseline(H, x, budget_s=12.0):
     fixed = max(t1 - per_col * n1, 0.0)
     per_matvec = fixed + per_col * D
     return {"value": 1.0 / per_matvec, "unit": "H*psi/s", "cores": nthreads, "kind": "port",
+            "host_nproc": os.cpu_count(),
             "sample": f"{n3} of {D} evenly spaced columns of one L=32 TP k=0 p=+1 ComplexF64 "
                       f"H*psi (scatter colmn! per column, threaded with private buffers) took "
                       f"{t3:.1f} s; fixed {fixed:.2f} s + {per_col * 1e6:.2f} us/column -> "
@@ -465,7 +495,7 @@ def c2_workload(args, ev):
     }
     if not args.no_cpu:
         from oracle import oracle as orc
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = host_threads()
         ob = orc.Basis("translational", 24, N=12, k=0, nthreads=nthreads)
         op = orc.Op(ob, H.flatten())
         t0 = time.perf_counter()
@@ -518,7 +548,7 @@ def c1_workload(args, ev):
     }
     if not args.no_cpu:
         from oracle import oracle as orc
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = host_threads()
         ob = orc.Basis("tensor", 16)
         op = orc.Op(ob, H.flatten())
         tgt = np.zeros(D, np.complex128)
@@ -702,7 +732,7 @@ def c7_workload(args, ev):
     }
     if not args.no_cpu:
         from oracle import oracle as orc
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = host_threads()
         t0 = time.perf_counter()
         ob = orc.Basis("abelian", Lt, N=N, symmetries=sym, nthreads=nthreads)
         t_ob = time.perf_counter() - t0

@@ -778,8 +778,6 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total, 
 // host: eligibility and setup (cached on the operator)
 // ---------------------------------------------------------------------------
 struct FastState {
-  bool checked = false;
-  bool ok = false;
   FastArgs args;
   FastTab *d_tab = nullptr;
   uint32_t *d_keys32 = nullptr;

@@ -510,11 +510,23 @@ static bool find_families(const edk_op *op, FastTab &tab) {
 
 static std::mutex g_setup_mu;  // first-time setup may be reached from several threads
 
+static bool fast_setup_build(edk_op *op);
+
+// Launches read the published state without the lock; only a launch that
+// finds setup not yet done takes the mutex (and waits for a build running in
+// another thread).  The single store of fstate publishes the state.
 static bool fast_setup(edk_op *op) {
+  const int st = op->fstate.load(std::memory_order_acquire);
+  if (st) return st > 0;
   std::lock_guard<std::mutex> setup_lock(g_setup_mu);
+  if (op->fstate.load(std::memory_order_relaxed)) return op->fstate.load(std::memory_order_relaxed) > 0;
+  const bool ok = fast_setup_build(op);
+  op->fstate.store(ok ? 1 : -1, std::memory_order_release);
+  return ok;
+}
+
+static bool fast_setup_build(edk_op *op) {
   FastState *S = fast_state(op);
-  if (S->checked) return S->ok;
-  S->checked = true;
   const edk_basis *b = op->basis;
   if (b->base != 2 || b->L > 32 || b->uploaded) return false;
   if (b->kind != EDK_TRANSLATION_PARITY && b->kind != EDK_TRANSLATIONAL && b->kind != EDK_PROJECTED) return false;
@@ -532,13 +544,11 @@ static bool fast_setup(edk_op *op) {
   A.ndiv = 0;
   for (int p = 1; p < b->len && A.ndiv < 16; p++)
     if (b->len % p == 0) A.divs[A.ndiv++] = p;
-  // tuning knobs (defaults picked by measurement; the env vars override)
-  const char *kb = getenv("EDK_FAST_KEYS_PER_BUCKET");
-  const char *sc = getenv("EDK_FAST_SCAN");
-  int keys_per_bucket = kb ? atoi(kb) : 1;
-  A.scan = sc ? atoi(sc) : 0;
-  const char *bt = getenv("EDK_FAST_BATCH");
-  A.batch = bt ? atoi(bt) : 2;
+  // lane schedule: targets in flight per lane (the variants override it);
+  // directory: about one slot per representative; lookups binary-search
+  const int keys_per_bucket = 1;
+  A.scan = 0;
+  A.batch = 2;
   S->cplx = (b->kind == EDK_TRANSLATIONAL && !(b->kp == 0 || 2 * b->kp == b->len));
   edk_ctx *ctx = op->ctx;
   if (hipMalloc((void **)&S->d_tab, sizeof(FastTab)) != hipSuccess) return false;
@@ -734,7 +744,6 @@ static bool fast_setup(edk_op *op) {
   if (vb && vb[0] == '1')
     fprintf(stderr, "[edk fast] D=%u L=%d lookup=%s schedule=%s chunks=%u batch=%d\n", A.D, A.L,
             A.region ? "region" : (A.htab ? "hash" : "dir"), A.chunk ? "balanced" : "lane", A.nchunk, A.batch);
-  S->ok = true;
   return true;
 }
 

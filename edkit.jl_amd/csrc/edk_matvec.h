@@ -45,6 +45,7 @@ struct edk_op {
   uint32_t *d_gchunk = nullptr;  // generic tile kernel: first row of each chunk
   uint32_t ngchunk = 0;
   std::atomic<int> gstate{0};     // generic tile setup: 0 not yet, 1 built, -1 not applicable
+  std::atomic<int> fstate{0};     // specialised kernel setup: 0 not yet, 1 built, -1 not applicable
   size_t scratch_bytes = 0;
   std::vector<int> h_tdesc, h_mrow_off, h_rowptr, h_col;
   std::vector<double2> h_val;

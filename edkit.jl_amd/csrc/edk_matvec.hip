@@ -517,12 +517,20 @@ static std::mutex g_gtile_mu;  // first-time setup may be reached from several t
 // images of consecutive rows, which share cache lines, and it is faster
 // (J1-J2 ring, D = 10.4 M: 3.7 against 9.5 ms).  EDK_GENERIC_SCHEDULE=row or
 // =tile overrides the choice.  Allocates and synchronises: run before capture.
+static bool generic_setup_build(edk_op *op);
+
 bool edk_generic_setup(edk_op *op) {
   std::lock_guard<std::mutex> lk(g_gtile_mu);
   if (op->gstate) return op->gstate > 0;
-  // the chunks are published by the final store of gstate = 1; launches that
-  // see 0 (not yet) or -1 use the row kernel
-  op->gstate = -1;
+  // gstate stays 0 while the chunks are built, so a launch from another
+  // thread waits on the mutex for the result; the single store below
+  // publishes the chunks (1) or the row kernel (-1)
+  const bool ok = generic_setup_build(op);
+  op->gstate = ok ? 1 : -1;
+  return ok;
+}
+
+static bool generic_setup_build(edk_op *op) {
   const edk_basis *b = op->basis;
   const char *env = getenv("EDK_GENERIC_SCHEDULE");
   if (env && strcmp(env, "row") == 0) return false;
@@ -565,7 +573,6 @@ bool edk_generic_setup(edk_op *op) {
     return false;
   }
   op->ngchunk = (uint32_t)cs.size() - 1;
-  op->gstate = 1;
   return true;
 }
 

@@ -15,6 +15,7 @@
 // The spectrum: rho = M M^H (or M^H M, whichever is smaller) with rocBLAS
 // zherk, then its eigenvalues with rocSOLVER zheevd; they are the squared
 // singular values of M that ent_S takes (svdvals(...) .^ 2).
+#include <climits>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -212,6 +213,10 @@ extern "C" int edk_ent_spec(edk_basis *b, int dtype, const void *v, int nA, cons
   for (int q = 0; q < a.nB; q++) dB *= b->base;
   const int64_t dA = a.dA, n = std::min(dA, dB), k = std::max(dA, dB);
   if (n > 32768) return edk_fail(EDK_ERR_UNSUPPORTED, "edk_ent_spec: reduced density matrix above 32768^2");
+  // rocBLAS takes the inner size k and the leading dimension dA as 32-bit
+  // integers: reject before anything is allocated or built
+  if (k > (int64_t)INT32_MAX || dA > (int64_t)INT32_MAX)
+    return edk_fail(EDK_ERR_UNSUPPORTED, "edk_ent_spec: Schmidt matrix dimension above 2^31 - 1");
   int rc = EDK_OK;
   double2 *M = nullptr, *rho = nullptr;
   double *d_w = nullptr, *d_e = nullptr;

@@ -8,7 +8,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libedkit_hip.so")
+# EDK_HIP_LIB may name another build of the same library (for A/B timing of
+# kernel variants); unset, the in-tree build is loaded
+LIB_PATH = os.environ.get("EDK_HIP_LIB") or os.path.join(_HERE, "libedkit_hip.so")
 
 EDK_OK = 0
 (EDK_TENSOR, EDK_PROJECTED, EDK_TRANSLATIONAL, EDK_TRANSLATION_PARITY, EDK_ABELIAN, EDK_PARITY, EDK_FLIP,

@@ -110,3 +110,20 @@ def test_ent_spec_at_l20(gpu):
     s2 = edk.ent_spec(v, A, B) ** 2
     assert np.abs(s2 - so).max() <= 1e-12
     assert abs(edk.ent_S(v, A, B) - edk.entropy(so)) <= 1e-11
+
+
+def test_ent_spec_rejects_32bit_overflow(gpu):
+    """L = 32 with one site in A: the inner size 2^31 of rho_A = M M^H does not
+    fit rocBLAS's 32-bit sizes, so the call is refused before any allocation
+    (the dummy vector is never read)."""
+    import ctypes as C
+    import edkit_hip as edk
+    from edkit_hip._lib import lib
+    B = edk.TensorBasis(L=32)
+    dummy = edk.DeviceArray((4,), np.complex128)
+    A = np.array([1], np.int32)
+    s2 = np.zeros(2, np.float64)
+    cnt = C.c_int64(-1)
+    rc = lib().edk_ent_spec(B.h, 0, dummy.ptr, 1, A.ctypes.data, s2.ctypes.data, C.byref(cnt))
+    assert rc != 0 and cnt.value == -1
+    assert "2^31" in lib().edk_last_error().decode()
