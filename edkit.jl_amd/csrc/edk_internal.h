@@ -26,6 +26,8 @@
 // ---------------------------------------------------------------------------
 void edk_set_error(const std::string &msg);
 int edk_fail(int code, const std::string &msg);
+// device address of edk_vec_dot_noalloc's scratch array, once per device (device current)
+int edk_noalloc_setup(int device);
 #define EDK_HIP(call)                                                        \
   do {                                                                       \
     hipError_t e_ = (call);                                                  \

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "edk_krylov.h"
@@ -1359,3 +1360,66 @@ extern "C" int edk_vec_nrm2(edk_ctx *ctx, int dtype, const void *x, int64_t n, d
   *out = v;
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free x^H y for callers that cannot allocate device memory (no
+// counterpart in the reference).  The block partials and the reduced value
+// live in one file-scope device array per GPU: 1024 partials (the most
+// red_blocks() ever makes) and one result, 16.4 KB held until the process
+// ends.  Its device address is looked up once per device, with the device
+// current, when a context is created there; kernels and copies take that
+// address, never the array's host-side name.  Every call on a GPU shares the
+// array, so calls that overlap on one GPU (other streams, other threads) race.
+// Nothing in it needs to start at zero: the partial pass writes every slot the
+// reduction reads.
+// ---------------------------------------------------------------------------
+#define EDK_NOALLOC_SLOTS 1025
+#define EDK_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double2 scratch[EDK_NOALLOC_SLOTS];
+static std::mutex g_scratch_mu;
+static double2 *g_scratch_ptr[EDK_NOALLOC_MAX_DEVICES] = {};
+
+int edk_noalloc_setup(int device) {
+  if (device < 0 || device >= EDK_NOALLOC_MAX_DEVICES) return EDK_OK;  // the entry point refuses such devices
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return EDK_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return edk_fail(EDK_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = (double2 *)p;
+  return EDK_OK;
+}
+
+extern "C" int edk_vec_dot_noalloc(edk_ctx *ctx, int dtype, const void *x, const void *y, int64_t n, double *re,
+                                   double *im) {
+  if (!ctx || !re || !im || n < 0) return edk_fail(EDK_ERR_ARG, "edk_vec_dot_noalloc: bad argument");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "edk_vec_dot_noalloc: bad dtype");
+  if (n > 0 && (!x || !y)) return edk_fail(EDK_ERR_ARG, "edk_vec_dot_noalloc: null vector");
+  const int nblk = red_blocks(n);
+  if (nblk + 1 > EDK_NOALLOC_SLOTS)
+    return edk_fail(EDK_ERR_UNSUPPORTED, "edk_vec_dot_noalloc: more block partials than the 1025-slot scratch");
+  if (ctx->device < 0 || ctx->device >= EDK_NOALLOC_MAX_DEVICES)
+    return edk_fail(EDK_ERR_UNSUPPORTED, "edk_vec_dot_noalloc: device ordinal above 63");
+  double2 *ptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ptr = g_scratch_ptr[ctx->device];
+  }
+  if (!ptr) return edk_fail(EDK_ERR_STATE, "edk_vec_dot_noalloc: scratch address not set up on this device");
+  *re = 0.0;
+  *im = 0.0;
+  if (n == 0) return EDK_OK;
+  EDK_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  DISPATCH(VT, k_dot_part<VT><<<nblk, RED_TPB, 0, s>>>((const VT *)x, (const VT *)y, n, ptr));
+  k_reduce_part<<<1, RED_TPB, 0, s>>>(ptr, nblk, ptr + (EDK_NOALLOC_SLOTS - 1));
+  EDK_HIP(hipGetLastError());
+  double2 *h = (double2 *)ctx->pinned;
+  EDK_HIP(hipMemcpyAsync(h, ptr + (EDK_NOALLOC_SLOTS - 1), sizeof(double2), hipMemcpyDeviceToHost, s));
+  EDK_HIP(hipStreamSynchronize(s));
+  *re = h[0].x;
+  *im = h[0].y;
+  return EDK_OK;
+}
+

@@ -46,6 +46,14 @@ extern "C" int edk_ctx_create(int device, void *stream, edk_ctx **out) {
     delete c;
     return edk_fail(EDK_ERR_OOM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
   }
+  // the device address of the allocation-free entry point's scratch array,
+  // once per device, with this device current
+  if (int rc = edk_noalloc_setup(device)) {
+    (void)hipHostFree(c->pinned);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return EDK_OK;
 }

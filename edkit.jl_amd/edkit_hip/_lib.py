@@ -34,7 +34,7 @@ SYMBOLS = [
     "edk_batch_create", "edk_batch_launch", "edk_batch_destroy",
     "edk_server_create", "edk_server_submit", "edk_server_wait", "edk_server_clock",
     "edk_server_stats", "edk_server_destroy",
-    "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2",
+    "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2", "edk_vec_dot_noalloc",
     "edk_lanczos_ground",
     "edk_lanczos_ground_ex",
     "edk_schmidt", "edk_ent_spec",
@@ -177,6 +177,7 @@ def lib():
         "edk_vec_axpy": (I32, [P, I32, P, P, D, D, I64]),
         "edk_vec_dot": (I32, [P, I32, P, P, I64, C.POINTER(D), C.POINTER(D)]),
         "edk_vec_nrm2": (I32, [P, I32, P, I64, C.POINTER(D)]),
+        "edk_vec_dot_noalloc": (I32, [P, I32, P, P, I64, C.POINTER(D), C.POINTER(D)]),
         "edk_lanczos_ground": (I32, [P, I32, P, I32, D, I32, C.POINTER(D), C.POINTER(D),
                                      C.POINTER(C.c_int), P]),
         "edk_lanczos_ground_ex": (I32, [P, I32, P, I32, D, I32, I32, C.POINTER(D), C.POINTER(D),

@@ -226,6 +226,14 @@ int edk_vec_axpy(edk_ctx *ctx, int dtype, void *y, const void *x, double a_re, d
 int edk_vec_dot(edk_ctx *ctx, int dtype, const void *x, const void *y, int64_t n, double *re,
                 double *im);                                                              /* x^H y */
 int edk_vec_nrm2(edk_ctx *ctx, int dtype, const void *x, int64_t n, double *out);         /* ||x|| */
+/* x^H y like edk_vec_dot, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  The block partials live in a library-
+ * owned 1025-slot (16.4 KB) device array per GPU, so any n >= 0 fits; calls
+ * that overlap on one GPU (several streams or threads) race on it.  Same
+ * kernels and reduction order as edk_vec_dot: bitwise the same result. */
+int edk_vec_dot_noalloc(edk_ctx *ctx, int dtype, const void *x, const void *y, int64_t n, double *re,
+                        double *im);
 
 /* ---- Schmidt decomposition of a sector state ------------------------------- */
 /* schmidt(v, Ainds, b) (Schmidt.jl:186-330, AbelianBasis.jl:704-719): the
