@@ -337,15 +337,29 @@ def c5_stream(n, rate, seed):
     return jobs
 
 
-def c5_workload(args, ev):
-    """Serving stream through the server (continuous admission, `workers`
-    streams) versus the same jobs solved one at a time.  Sectors and
-    operators are built before timing; the one-at-a-time pass runs first
-    (fresh memory pool), then the served pass."""
+def _pctl(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def c5_run(n_requests, rate, workers, wait_us=250.0, legs=("plain", "single", "served"), log=None):
+    """The serving stream three ways, on the same seeded jobs, operators and
+    product-state starts:
+    - plain: one request at a time through the direct entry points with the
+      reference's algorithms (Lanczos with full reorthogonalisation for the
+      ground state, KrylovEvolutionCache with full reorthogonalisation),
+      ComplexF64 -- the one-at-a-time plain GPU solves;
+    - single: one request at a time through the server (one worker) with the
+      serving algorithms (partial reorthogonalisation, Simon 1984);
+    - served: the server with `workers` threads, requests submitted at their
+      arrival times, matvecs of jobs sharing a sector and term structure
+      batched step by step (continuous batching).
+    Latency is arrival -> result; for the one-at-a-time legs it is the FIFO
+    queue of the same arrivals served back to back."""
     import edkit_hip as edk
+    from edkit_hip._lib import check, lib
     sys.path.insert(0, ROOT)
     from tests.models import blbq, j1j2, xxz
-    jobs = c5_stream(args.requests, args.rate, 5)
+    jobs = c5_stream(n_requests, rate, 5)
     bases = {}
 
     def basis_of(j):
@@ -363,14 +377,10 @@ def c5_workload(args, ev):
         name, c = j["model"]
         return xxz(B, c) if name == "xxz" else (j1j2(B, c) if name == "j1j2" else blbq(B, c))
 
-    def submit(srv, j, H):
-        idx = min(int(j["start"] * H.B.D), H.B.D - 1)
-        if j["kind"] == "ground":
-            return srv.submit_ground(H, psi0_index=idx, m=400, tol=1e-9, reorth=args.reorth)
-        return srv.submit_evolve(H, j["t"], psi0_index=idx)
+    def say(msg):
+        if log:
+            print(msg, file=log, flush=True)
 
-    # every sector and operator is built (and its matvec tables set up by one
-    # product) before either timed phase, so both phases time the solves only
     t_build = time.perf_counter()
     ops = [operator_of(j) for j in jobs]
     for H in {id(H): H for H in ops}.values():
@@ -378,73 +388,149 @@ def c5_workload(args, ev):
     for B in bases.values():
         B.ctx.synchronize()
     t_build = time.perf_counter() - t_build
-    # one at a time first (same jobs and operators), then served
-    seq_res = []
-    with edk.Server(workers=1) as srv:
-        s0 = srv.clock()
-        for i, (j, H) in enumerate(zip(jobs, ops)):
-            r, _ = srv.wait(submit(srv, j, H))
-            seq_res.append(r)
+    starts = [min(int(j["start"] * H.B.D), H.B.D - 1) for j, H in zip(jobs, ops)]
+    arrivals = np.array([j["arrival"] for j in jobs])
+    out = {"requests": len(jobs), "sectors_built": len(bases), "sector_build_s": t_build}
+
+    def fifo_latency(service):
+        done, lat = 0.0, []
+        for a, sv in zip(arrivals, service):
+            done = max(done, a) + sv
+            lat.append(done - a)
+        return lat
+
+    results = {}
+    if "plain" in legs:
+        ctx = edk.default_context()
+        service, res = [], []
+        for i, (j, H, idx) in enumerate(zip(jobs, ops, starts)):
+            x0 = np.zeros(H.B.D, np.complex128)
+            x0[idx] = 1.0
+            x0 = edk.DeviceArray.from_host(x0, ctx=H.B.ctx)
+            H.B.ctx.synchronize()
+            t0 = time.perf_counter()
+            if j["kind"] == "ground":
+                E, r, steps, _ = edk.lanczos_ground(H, x0, m=400, tol=1e-9, refine=False, reorth="full")
+                res.append({"E0": E, "resid": r, "steps": steps})
+            else:
+                c = edk.KrylovEvolutionCache(H, x0, tol=1e-10, m_init=30, m_max=60, reorth="full")
+                y = c.evolve(j["t"])
+                re_, im_ = C.c_double(), C.c_double()
+                check(lib().edk_vec_dot(H.B.ctx.h, 0, x0.ptr, y.ptr, H.B.D, C.byref(re_), C.byref(im_)))
+                nrm = C.c_double()
+                check(lib().edk_vec_nrm2(H.B.ctx.h, 0, y.ptr, H.B.D, C.byref(nrm)))
+                res.append({"overlap": complex(re_.value, im_.value), "norm": nrm.value})
+                del c, y
+            service.append(time.perf_counter() - t0)
             if (i + 1) % 50 == 0:
-                print(f"[c5] one-at-a-time {i + 1}/{len(jobs)}", file=sys.stderr, flush=True)
-        seq_span = srv.clock() - s0
-    seq = len(jobs) / seq_span
-    # served: requests submitted at their arrival times
-    with edk.Server(workers=args.workers) as srv:
-        t0 = srv.clock()
-        tickets = []
-        for j, H in zip(jobs, ops):
-            now = srv.clock() - t0
-            if j["arrival"] > now:
-                time.sleep(j["arrival"] - now)
-            tickets.append(submit(srv, j, H))
-        res = []
-        for i, tk in enumerate(tickets):
-            r, _ = srv.wait(tk)
-            res.append(r)
-            if (i + 1) % 50 == 0:
-                print(f"[c5] served {i + 1}/{len(tickets)}", file=sys.stderr, flush=True)
-        st = srv.stats()
-    bad = [r["message"] for r in res if r["status"] != 0]
-    lat = np.array([r["t_end"] - t0 - j["arrival"] for r, j in zip(res, jobs)])
-    span = max(r["t_end"] for r in res) - t0
-    served = len(jobs) / span
-    same = all(a["E0"] == b["E0"] and a["overlap_re"] == b["overlap_re"] and a["overlap_im"] == b["overlap_im"]
-               for a, b in zip(res, seq_res))
-    ground = [r for r, j in zip(res, jobs) if j["kind"] == "ground"]
-    # where the one-at-a-time time goes: per (L, kind) count, mean job seconds,
-    # mean matvecs and share of the total
-    classes = {}
-    for r, j in zip(seq_res, jobs):
-        c = classes.setdefault(f"L{j['L']}{'s1' if j['base'] == 3 else ''}-{j['kind']}", [0, 0.0, 0, []])
-        c[0] += 1
-        c[1] += r["t_end"] - r["t_start"]
-        c[2] += r["matvecs"]
-        c[3].append(r["t_end"] - r["t_start"])
-    tot = sum(c[1] for c in classes.values())
-    breakdown = {k: {"n": c[0], "mean_s": c[1] / c[0], "median_s": float(np.median(c[3])),
-                     "mean_matvecs": c[2] / c[0], "share": c[1] / tot}
-                 for k, c in sorted(classes.items(), key=lambda kv: -kv[1][1])}
+                say(f"[c5] plain {i + 1}/{len(jobs)}")
+        results["plain"] = res
+        lat = fifo_latency(service)
+        out["plain"] = {"req_s": len(jobs) / sum(service), "busy_s": sum(service),
+                        "latency_s": {"p50": _pctl(lat, 50), "p99": _pctl(lat, 99)}}
+        classes = {}
+        for j, sv in zip(jobs, service):
+            k = f"L{j['L']}{'s1' if j['base'] == 3 else ''}-{j['kind']}"
+            c = classes.setdefault(k, [0, 0.0])
+            c[0] += 1
+            c[1] += sv
+        tot = sum(service)
+        out["plain"]["time_share"] = {k: round(v[1] / tot, 4) for k, v in
+                                      sorted(classes.items(), key=lambda kv: -kv[1][1])[:8]}
+
+    def submit(srv, j, H, idx):
+        if j["kind"] == "ground":
+            return srv.submit_ground(H, psi0_index=idx, m=400, tol=1e-9, reorth="partial")
+        return srv.submit_evolve(H, j["t"], psi0_index=idx, reorth="partial")
+
+    def pack(r, j):
+        if j["kind"] == "ground":
+            return {"E0": r["E0"], "resid": r["resid"], "steps": r["steps"]}
+        return {"overlap": complex(r["overlap_re"], r["overlap_im"]), "norm": r["norm"]}
+
+    if "single" in legs:
+        with edk.Server(workers=1, batch_wait_us=wait_us) as srv:
+            s0 = srv.clock()
+            res, service = [], []
+            for i, (j, H, idx) in enumerate(zip(jobs, ops, starts)):
+                r, _ = srv.wait(submit(srv, j, H, idx))
+                res.append(pack(r, j))
+                service.append(r["t_end"] - r["t_start"])
+                if (i + 1) % 50 == 0:
+                    say(f"[c5] single {i + 1}/{len(jobs)}")
+            span = srv.clock() - s0
+        results["single"] = res
+        lat = fifo_latency(service)
+        out["single"] = {"req_s": len(jobs) / span, "latency_s": {"p50": _pctl(lat, 50), "p99": _pctl(lat, 99)}}
+
+    if "served" in legs:
+        with edk.Server(workers=workers, batch_wait_us=wait_us) as srv:
+            t0 = srv.clock()
+            tickets = []
+            for j, H, idx in zip(jobs, ops, starts):
+                now = srv.clock() - t0
+                if j["arrival"] > now:
+                    time.sleep(j["arrival"] - now)
+                tickets.append(submit(srv, j, H, idx))
+            raw = []
+            for i, tk in enumerate(tickets):
+                r, _ = srv.wait(tk)
+                raw.append(r)
+                if (i + 1) % 50 == 0:
+                    say(f"[c5] served {i + 1}/{len(tickets)}")
+            st = srv.stats()
+            launches, columns, wcols = srv.batch_stats(weighted=True)
+        bad = [r["message"] for r in raw if r["status"] != 0]
+        lat = [r["t_end"] - t0 - a for r, a in zip(raw, arrivals)]
+        span = max(r["t_end"] for r in raw) - t0
+        results["served"] = [pack(r, j) for r, j in zip(raw, jobs)]
+        out["served"] = {"req_s": len(jobs) / span, "latency_s": {"p50": _pctl(lat, 50), "p99": _pctl(lat, 99)},
+                         "failed": len(bad), "first_error": bad[0] if bad else None,
+                         "workers": workers, "peak_concurrent_jobs": st["peak_running"],
+                         "peak_inflight_GB": st["peak_inflight_bytes"] / 1e9,
+                         "batched_launches": launches,
+                         "columns_per_launch": columns / launches if launches else None,
+                         "columns_per_launch_dim_weighted": wcols,
+                         "max_ground_resid": max((r["resid"] for r, j in zip(raw, jobs) if j["kind"] == "ground"),
+                                                 default=None)}
+    # accuracy across legs: energies and evolved overlaps against the plain solves
+    if "plain" in results:
+        for leg in ("single", "served"):
+            if leg not in results:
+                continue
+            dE = [abs(a["E0"] - b["E0"]) for a, b, j in zip(results[leg], results["plain"], jobs)
+                  if j["kind"] == "ground"]
+            dO = [abs(a["overlap"] - b["overlap"]) for a, b, j in zip(results[leg], results["plain"], jobs)
+                  if j["kind"] == "evolve"]
+            out[leg]["max_dE_vs_plain"] = max(dE, default=None)
+            out[leg]["max_overlap_diff_vs_plain"] = max(dO, default=None)
+    if "single" in results and "served" in results:
+        out["served"]["identical_to_single"] = all(a == b for a, b in zip(results["single"], results["served"]))
+    if "plain" in out and "served" in out:
+        out["speedup_served_vs_plain"] = out["served"]["req_s"] / out["plain"]["req_s"]
+    if "single" in out and "served" in out:
+        out["speedup_served_vs_single"] = out["served"]["req_s"] / out["single"]["req_s"]
+    return out
+
+
+def c5_workload(args, ev):
+    """Serving stream (Poisson arrivals, mixed sectors and Hamiltonians,
+    ground-state and evolution requests): the served rate with continuous
+    batching against one-at-a-time plain solves (c5_run)."""
+    legs = ("plain", "single", "served") if not args.no_plain else ("single", "served")
+    r = c5_run(args.requests, args.rate, args.workers, args.batch_wait_us, legs, log=sys.stderr)
     out = {
         "metric": "mixed-solve req/s (served) with p50/p99 latency",
-        "value": served, "unit": "req/s", "n_gpus": 1, "steps": len(jobs), "warmup": 0,
-        "ms_per_step": 1e3 / served, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "complex128", "data": "synthetic (seeded Poisson stream, product-state starts)",
-        "config": {"workload": f"serving stream: {len(jobs)} requests, Poisson {args.rate} req/s, spin-1/2 "
+        "value": r["served"]["req_s"], "unit": "req/s", "n_gpus": 1, "steps": r["requests"], "warmup": 0,
+        "ms_per_step": 1e3 / r["served"]["req_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex128", "data": "synthetic (seeded Poisson stream, product-state starts)",
+        "config": {"workload": f"serving stream: {r['requests']} requests, Poisson {args.rate} req/s, spin-1/2 "
                                "L=18..30 XXZ/J1-J2 and spin-1 BLBQ L=12..16, random k/p sectors, Sz=0; "
                                "60% Lanczos to residual 1e-9, 40% Krylov evolution to t in [1,20]",
-                   "workers": args.workers, "ground_reorth": args.reorth, "parallelism": "single GPU"},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "mean": float(lat.mean())},
-        "one_at_a_time_req_s": seq,
-        "speedup_vs_one_at_a_time": served / seq,
-        "results_identical_alone_and_served": bool(same),
-        "failed": len(bad), "first_error": bad[0] if bad else None,
-        "max_ground_resid": max((r["resid"] for r in ground), default=None),
-        "peak_concurrent_jobs": st["peak_running"], "peak_inflight_GB": st["peak_inflight_bytes"] / 1e9,
-        "sectors_built": len(bases), "sector_build_s": t_build,
-        "one_at_a_time_breakdown": breakdown,
+                   "workers": args.workers, "parallelism": "single GPU"},
+        "latency_s": r["served"]["latency_s"],
     }
+    out.update({k: v for k, v in r.items() if k not in ("requests",)})
     print(json.dumps(out))
 
 
@@ -767,9 +853,12 @@ def main():
                          "propagation d=1024; c7: 2D torus with custom generators")
     ap.add_argument("--requests", type=int, default=2000, help="c5: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate, req/s")
-    ap.add_argument("--workers", type=int, default=8, help="c5: server worker streams")
-    ap.add_argument("--reorth", default="partial", choices=["full", "partial"],
-                    help="c5: reorthogonalisation of the ground-state jobs")
+    ap.add_argument("--workers", type=int, default=32, help="c5: server worker streams")
+    ap.add_argument("--batch-wait-us", type=float, default=250.0,
+                    help="c5: longest wait for a group's other members before a partial block launches")
+    ap.add_argument("--no-plain", action="store_true", help="c5: skip the plain one-at-a-time leg")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="c3 line: requests of its serving leg (0 skips it)")
     ap.add_argument("--lanczos-steps", type=int, default=300,
                     help="steps of the ground-state Lanczos leg (0 skips it)")
     args = ap.parse_args()
@@ -851,6 +940,22 @@ def main():
         out["c32"] = {"value": 1e3 / s32, "kernel_ms": k32, "achieved_GBs": b32 / (k32 * 1e-3) / 1e9}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(H, xh)
+    if args.serve_requests > 0:
+        # the metric's second half: a reduced serving stream (c5_run)
+        del x, y
+        r = c5_run(args.serve_requests, args.rate, args.workers, args.batch_wait_us, log=sys.stderr)
+        out["serving"] = {
+            "metric": "mixed-solve req/s (served, continuous batching) with p50/p99 latency",
+            "requests": r["requests"], "req_s": r["served"]["req_s"], "latency_s": r["served"]["latency_s"],
+            "plain_one_at_a_time_req_s": r["plain"]["req_s"], "plain_latency_s": r["plain"]["latency_s"],
+            "single_served_req_s": r["single"]["req_s"],
+            "speedup_vs_plain": r["speedup_served_vs_plain"], "speedup_vs_single": r["speedup_served_vs_single"],
+            "columns_per_launch": r["served"]["columns_per_launch"], "failed": r["served"]["failed"],
+            "max_ground_resid": r["served"]["max_ground_resid"],
+            "max_dE_vs_plain": r["served"]["max_dE_vs_plain"],
+            "max_overlap_diff_vs_plain": r["served"]["max_overlap_diff_vs_plain"],
+            "identical_to_single": r["served"]["identical_to_single"],
+        }
     print(json.dumps(out))
 
 

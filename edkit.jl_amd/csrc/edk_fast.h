@@ -80,15 +80,12 @@ struct FastArgs {
   uint32_t ndir;
   uint32_t D;
   const double *Rrow;
-  const uint16_t *perm;  // per tile of FAST_TILE rows: row order by target count (or null)
   const uint2 *region;   // two-level directory: per region (key >> rshift) {base index, fine offset << 5 | shift}
   int rshift;            // region width in key bits
   const uint16_t *fine;  // per region: positions relative to base, one slot per 2^shift key values
   const uint32_t *chunk; // balanced schedule: first row of each chunk (nchunk + 1 entries), or null
   uint32_t nchunk;
-  const uint4 *htab;     // hash index: buckets of 8 (key, position) slots, or null
-  uint32_t hbuckets;     // number of buckets
-  int hprobe;            // longest probe sequence in buckets (>= 1)
+  uint32_t xcdg;         // balanced schedule: consecutive chunks given to one XCD (0: plain order)
 };
 
 __device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
@@ -597,26 +594,8 @@ __device__ __forceinline__ void canon32_brute(uint32_t s, uint32_t &rep, uint32_
   code = (minA == minB ? 32 : 0) + period - 1;
 }
 
+// marks a target whose representative is not in the table
 #define HASH_EMPTY 0xffffffffu
-__device__ __forceinline__ uint32_t hash_bucket(uint32_t key, uint32_t nb) {
-  return (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
-}
-
-// One bucket of the hash index: 8 (key, position) slots in 64 bytes.
-// Returns the position, HASH_EMPTY when the bucket proves the key absent
-// (it has a free slot), or HASH_EMPTY - 1 when the next bucket must be read.
-__device__ __forceinline__ uint32_t hash_probe(const uint4 *b, uint32_t key) {
-  uint4 w0 = b[0], w1 = b[1], w2 = b[2], w3 = b[3];
-  uint32_t r = HASH_EMPTY - 1;
-  bool empty = false;
-#define HSLOT(K, V)                         \
-  r = ((K) == key && (V) != HASH_EMPTY) ? (V) : r; \
-  empty = empty || (V) == HASH_EMPTY;
-  HSLOT(w0.x, w0.y) HSLOT(w0.z, w0.w) HSLOT(w1.x, w1.y) HSLOT(w1.z, w1.w)
-  HSLOT(w2.x, w2.y) HSLOT(w2.z, w2.w) HSLOT(w3.x, w3.y) HSLOT(w3.z, w3.w)
-#undef HSLOT
-  return (r == HASH_EMPTY - 1 && empty) ? HASH_EMPTY : r;
-}
 
 // Position of key in the sorted table, or -1.  The directory bounds the
 // search to [lo, hi); a bucket of at most A.scan keys is searched with one
@@ -700,14 +679,18 @@ __device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint
   uint2 rg[K];
 #pragma unroll
   for (int q = 0; q < K; q++) rg[q] = A.region[(ok[q] ? rep[q] : 0u) >> A.rshift];
-  uint32_t lo[K], hi[K];
+  // Only the bucket's first index is read.  The table is sorted, so every
+  // key from the bucket's start up to the target is below it and every key
+  // after it (later buckets, the padding) above it: among the four keys read,
+  // the count below the target is its offset when present.  When all four are
+  // below, the target may lie further on and the bucket's end is read.
+  uint32_t lo[K], fslot[K];
 #pragma unroll
   for (int q = 0; q < K; q++) {
     const uint32_t key = ok[q] ? rep[q] : 0u;
     const uint32_t sh = rg[q].y & 31, foff = rg[q].y >> 5;
-    const uint32_t fb = (key & ((1u << A.rshift) - 1)) >> sh;
-    lo[q] = rg[q].x + A.fine[foff + fb];
-    hi[q] = rg[q].x + A.fine[foff + fb + 1];
+    fslot[q] = foff + ((key & ((1u << A.rshift) - 1)) >> sh);
+    lo[q] = rg[q].x + A.fine[fslot[q]];
   }
   edk_u32x4 kv[K];
 #pragma unroll
@@ -715,20 +698,18 @@ __device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint
   bool any_bs = false;
 #pragma unroll
   for (int q = 0; q < K; q++) {
-    const uint32_t key = rep[q], n = hi[q] - lo[q];
-    const uint32_t below = (kv[q].x < key) + (n > 1 && kv[q].y < key) + (n > 2 && kv[q].z < key) +
-                           (n > 3 && kv[q].w < key);
-    const bool eq = (n > 0 && kv[q].x == key) || (n > 1 && kv[q].y == key) || (n > 2 && kv[q].z == key) ||
-                    (n > 3 && kv[q].w == key);
-    jj[q] = !ok[q] ? HASH_EMPTY : (n > FINE_SCAN ? EDK_NEED_BS : (eq ? lo[q] + below : HASH_EMPTY));
+    const uint32_t key = rep[q];
+    const uint32_t below = (kv[q].x < key) + (kv[q].y < key) + (kv[q].z < key) + (kv[q].w < key);
+    const bool eq = kv[q].x == key || kv[q].y == key || kv[q].z == key || kv[q].w == key;
+    jj[q] = !ok[q] ? HASH_EMPTY : (eq ? lo[q] + below : (below == 4 ? EDK_NEED_BS : HASH_EMPTY));
     any_bs = any_bs || jj[q] == EDK_NEED_BS;
   }
   if (any_bs) {
-    // buckets of more than FINE_SCAN keys (rare): binary search
+    // buckets of more than FINE_SCAN keys (rare): binary search over the rest
 #pragma unroll
     for (int q = 0; q < K; q++) {
       if (jj[q] == EDK_NEED_BS) {
-        uint32_t a = lo[q], b = hi[q], r = HASH_EMPTY;
+        uint32_t a = lo[q] + FINE_SCAN, b = rg[q].x + A.fine[fslot[q] + 1], r = HASH_EMPTY;
         while (a < b) {
           const uint32_t mid = (a + b) >> 1, v = A.keys[mid];
           if (v == rep[q]) {
@@ -752,6 +733,20 @@ __device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint
 // x, all BAL_K independent so they overlap.  Phase 3: each row thread sums its
 // contiguous slice of contributions in order and writes y.
 // ---------------------------------------------------------------------------
+// Chunk of workgroup b.  Workgroups are dispatched to the 8 XCDs round
+// robin, so in plain order neighbouring chunks, whose rows share many
+// targets, land on different L2 caches.  With G = xcdg, XCD x takes chunks in
+// runs of G: its k-th workgroup gets chunk (k / G) 8G + x G + k % G.  All XCDs
+// still advance through the rows together.  A bijection on the first
+// multiple of 8G chunks; the tail keeps plain order.
+__device__ __forceinline__ uint32_t bal_chunk(uint32_t b, uint32_t nchunk, uint32_t G) {
+  if (G <= 1) return b;
+  const uint32_t span = 8u * G, nfull = nchunk / span * span;
+  if (b >= nfull) return b;
+  const uint32_t x = b & 7u, k = b >> 3;
+  return (k / G) * span + x * G + (k % G);
+}
+
 #define BAL_TPB 256
 #define BAL_K 4
 #define BAL_CAP (BAL_TPB * BAL_K)
@@ -777,13 +772,32 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t *total, 
 // ---------------------------------------------------------------------------
 // host: eligibility and setup (cached on the operator)
 // ---------------------------------------------------------------------------
+// Values of one operator for the multi-coefficient block kernel: the entry
+// values and diagonals of its families (the structure -- distances, location
+// masks, flips -- is the group's, identical for every column).
+struct FastColTab {
+  double val[FAST_MAXFAM * FAST_MAXENT];
+  double diag[FAST_MAXFAM * 4];
+};
+#define MC_MAXV 8
+// Per-column arguments of k_matvec_bal_mc: column c applies the values of
+// ctab[c] to column c of the row-interleaved block and writes y[c] (a plain
+// vector of D elements; null: a padding column, not written).
+struct McArgs {
+  const FastColTab *ctab[MC_MAXV];
+  void *y[MC_MAXV];
+};
+struct ColPtrs {
+  const void *p[MC_MAXV];
+};
+
 struct FastState {
   FastArgs args;
   FastTab *d_tab = nullptr;
+  FastColTab *d_coltab = nullptr;  // this operator's values (multi-coefficient kernel)
+  uint64_t signature = 0;          // hash of the family structure (distances, masks, flips)
   uint32_t *d_keys32 = nullptr;
   uint32_t *d_dir = nullptr;
-  uint16_t *d_perm = nullptr;
-  uint4 *d_htab = nullptr;
   uint2 *d_region = nullptr;
   uint16_t *d_fine = nullptr;
   uint32_t *d_chunk = nullptr;
@@ -794,3 +808,9 @@ struct FastState {
 
 // block-of-vectors launch (edk_fast_blk.hip); nv in {2, 4, 8}
 bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, int nv, int acc, hipStream_t st);
+// multi-coefficient block launch (edk_fast_blk.hip): nv in {1, 2, 4, 8}
+// columns of the row-interleaved block X (D x nv), column c with the values of
+// M.ctab[c], written to M.y[c]; A is any member's arguments (shared structure)
+bool edk_fast_launch_mc(const FastArgs &A, const McArgs &M, int dtype, const void *X, int nv, hipStream_t st);
+// X[i * nv + c] = (G.p[c])[i] for nv input vectors of D elements
+int edk_gather_cols(const ColPtrs &G, int nv, int64_t D, int dtype, void *X, hipStream_t st);

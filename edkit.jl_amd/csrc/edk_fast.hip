@@ -22,8 +22,6 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
   if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
   uint32_t i = blockIdx.x * FAST_TPB + threadIdx.x;
   if (i >= A.D) return;
-  // rows of a tile visited in order of their target count (optional schedule)
-  if (A.perm) i = (i & ~(FAST_TILE - 1)) + A.perm[i];
   const FastTab &T = *A.tab;
   const uint32_t s = A.keys[i];
   const uint32_t ns = ~s & A.mask;
@@ -66,7 +64,7 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
           hi[q] = 0;
           if (ok[q] && A.region) {
             region_bounds(A, rep[q], lo[q], hi[q]);
-          } else if (ok[q] && !A.htab) {
+          } else if (ok[q]) {
             uint32_t bkt = rep[q] >> A.dshift;
             if (bkt < A.ndir) {
               lo[q] = A.dir[bkt];
@@ -80,24 +78,6 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
         if (A.region) {
 #pragma unroll
           for (int q = 0; q < NB; q++) jj[q] = region_find(A, rep[q], lo[q], hi[q]);
-        } else if (A.htab) {
-          // home buckets of all targets first (independent loads), then the
-          // rare overflow into the following buckets
-          uint32_t hb[NB];
-#pragma unroll
-          for (int q = 0; q < NB; q++) {
-            hb[q] = hash_bucket(rep[q], A.hbuckets);
-            jj[q] = ok[q] ? hash_probe(A.htab + 4 * (size_t)hb[q], rep[q]) : HASH_EMPTY;
-          }
-#pragma unroll
-          for (int q = 0; q < NB; q++) {
-            for (int t = 1; t < A.hprobe && jj[q] == HASH_EMPTY - 1; t++) {
-              uint32_t b2 = hb[q] + t;
-              if (b2 >= A.hbuckets) b2 -= A.hbuckets;
-              jj[q] = hash_probe(A.htab + 4 * (size_t)b2, rep[q]);
-            }
-            if (jj[q] == HASH_EMPTY - 1) jj[q] = HASH_EMPTY;
-          }
         } else {
 #pragma unroll
           for (int q = 0; q < NB; q++) jj[q] = (uint32_t)fast_find(A, rep[q], lo[q], hi[q]);
@@ -173,7 +153,8 @@ __global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && 
   FastArgs B = A;
   B.Rtab = sR;
   if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
-  const uint32_t r0 = A.chunk[blockIdx.x], nrow = A.chunk[blockIdx.x + 1] - r0;
+  const uint32_t cb = bal_chunk(blockIdx.x, A.nchunk, A.xcdg);
+  const uint32_t r0 = A.chunk[cb], nrow = A.chunk[cb + 1] - r0;
   // phase 1: row threads count and write their targets; the row's own x_i
   // and 1/R_i are loaded now so their latency hides behind phases 1 and 2
   uint32_t s = 0, n = 0;
@@ -429,10 +410,9 @@ void edk_fast_release(edk_op *op) {
   FastState *F = (FastState *)op->fast;
   if (!F) return;
   (void)hipFree(F->d_tab);
+  (void)hipFree(F->d_coltab);
   (void)hipFree(F->d_keys32);
   (void)hipFree(F->d_dir);
-  (void)hipFree(F->d_perm);
-  (void)hipFree(F->d_htab);
   (void)hipFree(F->d_region);
   (void)hipFree(F->d_fine);
   (void)hipFree(F->d_chunk);
@@ -553,6 +533,28 @@ static bool fast_setup_build(edk_op *op) {
   edk_ctx *ctx = op->ctx;
   if (hipMalloc((void **)&S->d_tab, sizeof(FastTab)) != hipSuccess) return false;
   if (hipMemcpyAsync(S->d_tab, &tab, sizeof(FastTab), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return false;
+  // this operator's values for the multi-coefficient kernel, and the hash of
+  // its structure (operators with equal hashes on one basis share launches)
+  {
+    FastColTab ct{};
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    mix((uint64_t)tab.nfam);
+    for (int f = 0; f < tab.nfam; f++) {
+      mix((uint64_t)tab.fam[f].d);
+      mix((uint64_t)tab.fam[f].nent);
+      for (int q = 0; q < 4; q++) ct.diag[f * 4 + q] = tab.fam[f].diag[q];
+      for (int e = 0; e < tab.fam[f].nent; e++) {
+        mix((uint64_t)tab.fam[f].ent[e].locs);
+        mix((uint64_t)tab.fam[f].ent[e].flip);
+        ct.val[f * FAST_MAXENT + e] = tab.fam[f].ent[e].val;
+      }
+    }
+    S->signature = h;
+    if (hipMalloc((void **)&S->d_coltab, sizeof(FastColTab)) != hipSuccess) return false;
+    if (hipMemcpyAsync(S->d_coltab, &ct, sizeof(FastColTab), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+      return false;
+  }
   // keys + 64 padding entries (the wide scan may read past the end)
   if (hipMalloc((void **)&S->d_keys32, (b->D + 64) * sizeof(uint32_t)) != hipSuccess) return false;
   if (hipMemsetAsync(S->d_keys32 + b->D, 0xff, 64 * sizeof(uint32_t), ctx->stream) != hipSuccess) return false;
@@ -574,14 +576,9 @@ static bool fast_setup_build(edk_op *op) {
   A.ndir = ndir;
   A.D = (uint32_t)b->D;
   A.Rrow = b->d_R;
-  A.perm = nullptr;
-  A.htab = nullptr;
-  A.hbuckets = 0;
-  A.hprobe = 1;
   A.region = nullptr;
   A.fine = nullptr;
-  const char *lk = getenv("EDK_FAST_LOOKUP");
-  if (!lk || strcmp(lk, "region") == 0) {
+  {
     // two-level directory (4096 regions of 2^20 key values); fine slots
     // ~ one per representative; entries are 16-bit offsets from the region's
     // first index, so a region may hold at most 65535 representatives
@@ -639,72 +636,13 @@ static bool fast_setup_build(edk_op *op) {
       }
     }
   }
-  if (lk && strcmp(lk, "hash") == 0) {
-    // hash index over the representatives: bucketised linear probing, built
-    // on the host in table order (deterministic), ~70% slot load
-    std::vector<uint32_t> keys(b->D);
-    if (hipMemcpyAsync(keys.data(), S->d_keys32, b->D * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-      return false;
-    uint32_t nb = (uint32_t)std::max<int64_t>(1, (int64_t)std::ceil(b->D / (8 * 0.7)));
-    std::vector<uint32_t> tab((size_t)nb * 16, 0xffffffffu);
-    int maxp = 1;
-    for (int64_t i = 0; i < b->D; i++) {
-      uint32_t key = keys[i];
-      uint32_t bk = (uint32_t)(((uint64_t)(key * 0x9E3779B1u) * nb) >> 32);
-      for (int t = 0;; t++) {
-        uint32_t *bb = &tab[(size_t)bk * 16];
-        int slot = -1;
-        for (int q = 0; q < 8 && slot < 0; q++)
-          if (bb[2 * q + 1] == 0xffffffffu) slot = q;
-        if (slot >= 0) {
-          bb[2 * slot] = key;
-          bb[2 * slot + 1] = (uint32_t)i;
-          maxp = std::max(maxp, t + 1);
-          break;
-        }
-        bk = (bk + 1 == nb) ? 0 : bk + 1;
-      }
-    }
-    if (hipMalloc((void **)&S->d_htab, tab.size() * 4) != hipSuccess) return false;
-    if (hipMemcpyAsync(S->d_htab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-      return false;
-    A.htab = S->d_htab;
-    A.hbuckets = nb;
-    A.hprobe = maxp;
-  }
-  // row order within tiles by target count (stable counting sort on the host)
-  const char *ps = getenv("EDK_FAST_SORT_ROWS");
-  if (ps && ps[0] == '1') {
-    uint8_t *d_cnt = nullptr;
-    std::vector<uint8_t> cnt(b->D);
-    std::vector<uint16_t> perm(b->D);
-    if (hipMalloc((void **)&d_cnt, b->D) != hipSuccess) return false;
-    k_row_targets<uint8_t><<<(unsigned)((b->D + 255) / 256), 256, 0, ctx->stream>>>(A, d_cnt);
-    bool ok = hipMemcpyAsync(cnt.data(), d_cnt, b->D, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
-              hipStreamSynchronize(ctx->stream) == hipSuccess;
-    (void)hipFree(d_cnt);
-    if (!ok) return false;
-    for (int64_t t0 = 0; t0 < b->D; t0 += FAST_TILE) {
-      int64_t t1 = std::min<int64_t>(b->D, t0 + FAST_TILE);
-      int hist[257] = {0};
-      for (int64_t i = t0; i < t1; i++) hist[cnt[i] + 1]++;
-      for (int c = 0; c < 256; c++) hist[c + 1] += hist[c];
-      for (int64_t i = t0; i < t1; i++) perm[t0 + hist[cnt[i]]++] = (uint16_t)(i - t0);
-    }
-    if (hipMalloc((void **)&S->d_perm, b->D * sizeof(uint16_t)) != hipSuccess) return false;
-    if (hipMemcpyAsync(S->d_perm, perm.data(), b->D * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-      return false;
-    A.perm = S->d_perm;
-  }
   // balanced schedule: chunks of whole rows with at most BAL_CAP targets
   A.chunk = nullptr;
   A.nchunk = 0;
-  const char *sch = getenv("EDK_FAST_SCHEDULE");
-  if (A.region && (!sch || strcmp(sch, "balanced") == 0)) {
+  // runs of 512 chunks per XCD (bal_chunk): against plain round-robin order
+  // 1.352 -> 1.294 ms at L = 32 (runs of 32: 1.307, 256: 1.297, 1024: 1.296)
+  A.xcdg = 512;
+  if (A.region) {
     // 16-bit counts: at most 3 targets per site, family and row (L <= 32)
     uint16_t *d_cnt = nullptr;
     std::vector<uint16_t> cnt(b->D);
@@ -740,10 +678,6 @@ static bool fast_setup_build(edk_op *op) {
       A.nchunk = (uint32_t)cs.size() - 1;
     }
   }
-  const char *vb = getenv("EDK_FAST_VERBOSE");
-  if (vb && vb[0] == '1')
-    fprintf(stderr, "[edk fast] D=%u L=%d lookup=%s schedule=%s chunks=%u batch=%d\n", A.D, A.L,
-            A.region ? "region" : (A.htab ? "hash" : "dir"), A.chunk ? "balanced" : "lane", A.nchunk, A.batch);
   return true;
 }
 
@@ -906,3 +840,44 @@ int edk_fast_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant,
   if (ms_out) *ms_out = best_ms;
   return EDK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Multi-coefficient launches (the server's batched matvecs)
+// ---------------------------------------------------------------------------
+bool edk_fast_signature(edk_op *op, uint64_t *sig) {
+  if (op->disable_fast || !fast_setup(op)) return false;
+  FastState *S = (FastState *)op->fast;
+  if (!S->args.chunk || !S->d_coltab) return false;
+  *sig = S->signature;
+  return true;
+}
+
+int edk_fast_matvec_mc(edk_op *const *ops, int n, int dtype, const void *const *xs, void *const *ys, void *Xblk,
+                       hipStream_t st) {
+  if (n < 1 || n > MC_MAXV) return edk_fail(EDK_ERR_ARG, "edk_fast_matvec_mc: 1..8 operators");
+  uint64_t sig0 = 0;
+  if (!edk_fast_signature(ops[0], &sig0)) return edk_fail(EDK_ERR_UNSUPPORTED, "edk_fast_matvec_mc: no fast path");
+  const int nv = n == 1 ? 1 : (n == 2 ? 2 : (n <= 4 ? 4 : 8));
+  McArgs M{};
+  ColPtrs G{};
+  for (int c = 0; c < nv; c++) {
+    edk_op *op = ops[c < n ? c : 0];
+    uint64_t sg = 0;
+    if (op->basis != ops[0]->basis || !edk_fast_signature(op, &sg) || sg != sig0)
+      return edk_fail(EDK_ERR_ARG, "edk_fast_matvec_mc: operators differ in basis or structure");
+    M.ctab[c] = ((FastState *)op->fast)->d_coltab;
+    M.y[c] = c < n ? ys[c] : nullptr;
+    G.p[c] = xs[c < n ? c : 0];
+  }
+  const FastArgs &A = ((FastState *)ops[0]->fast)->args;
+  // one column reads its vector in place; several are gathered into the block
+  const void *X = xs[0];
+  if (nv > 1) {
+    if (int rc = edk_gather_cols(G, nv, (int64_t)A.D, dtype, Xblk, st)) return rc;
+    X = Xblk;
+  }
+  if (!edk_fast_launch_mc(A, M, dtype, X, nv, st)) return edk_fail(EDK_ERR_UNSUPPORTED, "edk_fast_matvec_mc");
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+

@@ -40,7 +40,8 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
   FastArgs B = A;
   B.Rtab = sR;
   if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
-  const uint32_t r0 = A.chunk[blockIdx.x], nrow = A.chunk[blockIdx.x + 1] - r0;
+  const uint32_t cb = bal_chunk(blockIdx.x, A.nchunk, A.xcdg);
+  const uint32_t r0 = A.chunk[cb], nrow = A.chunk[cb + 1] - r0;
   // phase 1: row threads count and write their targets; diagonal and 1/R_i
   uint32_t s = 0, n = 0;
   const bool own = (uint32_t)tid < nrow;
@@ -244,3 +245,331 @@ bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, i
     default: return false;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Multi-coefficient block kernel: NV operators that share a basis and a
+// family structure (the same distances, location masks and flips) but not
+// their values -- e.g. XXZ chains of different Delta, or J1-J2 rings of
+// different J2, in one sector -- applied to the NV columns of one
+// row-interleaved block, column c with the values of operator c.  The
+// canonical form and the lookup of every target are computed once for all
+// columns; phase 3 gives each row NV lanes, and lane c applies its operator's
+// entry value and diagonal and writes its own output vector.  A column's
+// arithmetic does not depend on NV or on the other columns, so a vector gets
+// the same bits in a block of one as in any mix.
+// ---------------------------------------------------------------------------
+template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
+__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M, const VT *__restrict__ X) {
+  // every product and sum is rounded on its own (no fused multiply-add), so
+  // the one-column form below and the NV-column form round a column's terms
+  // identically: a vector gets the same bits alone and in any block
+#pragma clang fp contract(off)
+  constexpr bool CPLX = (PH == 2);
+  __shared__ double sR[2 * 64];
+  __shared__ double2 sP[PH != 0 ? 64 : 1];
+  __shared__ double sV[NV][FAST_MAXFAM * FAST_MAXENT];
+  __shared__ double sD[NV][FAST_MAXFAM * 4];
+  __shared__ uint32_t tgt[BAL_CAP];
+  __shared__ uint8_t ent[BAL_CAP];
+  // (position, canonical coefficient) per target; with one column the
+  // target's contribution (re, im) instead
+  __shared__ double tcr[BAL_CAP], tci[(CPLX || NV == 1) ? BAL_CAP : 1];
+  __shared__ uint32_t roff[BAL_TPB], rcnt[BAL_TPB];
+  __shared__ uint8_t rpop[BAL_TPB][FAST_MAXFAM * 4];
+  __shared__ ST rinv[BAL_TPB];
+  __shared__ uint32_t wsum[BAL_TPB / 64];
+  constexpr bool RUNS = (LT == 32 || LT == 1) && KIND != EDK_PROJECTED;
+  __shared__ uint16_t dq[RUNS ? BAL_CAP : 1];
+  __shared__ uint32_t ndef;
+  const FastTab &T = *A.tab;
+  const int tid = threadIdx.x;
+  if (RUNS && tid == 0) ndef = 0;
+  if (KIND != EDK_PROJECTED) {
+    int nr = (KIND == EDK_TRANSLATION_PARITY ? 2 : 1) * A.len;
+    if (tid < nr) sR[tid] = A.Rtab[tid];
+    if (PH != 0 && KIND == EDK_TRANSLATIONAL && tid < A.len) sP[tid] = A.ph[tid];
+  }
+  for (int q = tid; q < NV * FAST_MAXFAM * FAST_MAXENT; q += BAL_TPB)
+    sV[q / (FAST_MAXFAM * FAST_MAXENT)][q % (FAST_MAXFAM * FAST_MAXENT)] =
+        M.ctab[q / (FAST_MAXFAM * FAST_MAXENT)]->val[q % (FAST_MAXFAM * FAST_MAXENT)];
+  if (tid < NV * FAST_MAXFAM * 4) sD[tid / (FAST_MAXFAM * 4)][tid % (FAST_MAXFAM * 4)] =
+      M.ctab[tid / (FAST_MAXFAM * 4)]->diag[tid % (FAST_MAXFAM * 4)];
+  FastArgs B = A;
+  B.Rtab = sR;
+  if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
+  const uint32_t cb = bal_chunk(blockIdx.x, A.nchunk, A.xcdg);
+  const uint32_t r0 = A.chunk[cb], nrow = A.chunk[cb + 1] - r0;
+  // phase 1: row threads count and write their targets; the diagonal's
+  // popcounts (per family: 00, 01, 10, 11 pairs) and 1/R_i
+  uint32_t s = 0, n = 0;
+  const bool own = (uint32_t)tid < nrow;
+  if (own) {
+    s = A.keys[r0 + tid];
+    const uint32_t ns = ~s & A.mask;
+    for (int f = 0; f < T.nfam; f++) {
+      uint32_t r = rotr_lt<LT>(s, A.L - T.fam[f].d, A.L, A.mask);
+      for (int e = 0; e < T.fam[f].nent; e++) n += __popc(ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask));
+    }
+  }
+  uint32_t total;
+  const uint32_t off = block_scan_u32(n, &total, wsum);
+  if (own) {
+    const uint32_t ns = ~s & A.mask;
+    uint32_t o = off;
+    for (int f = 0; f < T.nfam; f++) {
+      const int d = T.fam[f].d;
+      uint32_t r = rotr_lt<LT>(s, A.L - d, A.L, A.mask);
+      uint32_t nr = ~r & A.mask;
+      rpop[tid][f * 4 + 0] = (uint8_t)__popc(ns & nr);
+      rpop[tid][f * 4 + 1] = (uint8_t)__popc(ns & r);
+      rpop[tid][f * 4 + 2] = (uint8_t)__popc(s & nr);
+      rpop[tid][f * 4 + 3] = (uint8_t)__popc(s & r);
+      for (int e = 0; e < T.fam[f].nent; e++) {
+        const int flip = T.fam[f].ent[e].flip;
+        uint32_t bits = ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask);
+        while (bits) {
+          int p = __ffs(bits) - 1;
+          bits &= bits - 1;
+          int p2 = p - d;
+          if (LT == 32) p2 &= 31;
+          else if (p2 < 0) p2 += A.L;
+          uint32_t t = s;
+          if (flip & 2) t ^= 1u << p;
+          if (flip & 1) t ^= 1u << p2;
+          tgt[o] = t;
+          ent[o] = (uint8_t)(f * FAST_MAXENT + e);
+          o++;
+        }
+      }
+    }
+    roff[tid] = off;
+    rcnt[tid] = n;
+    rinv[tid] = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[r0 + tid]);
+  }
+  __syncthreads();
+  // phase 2: BAL_K targets per thread -> (position, canonical coefficient)
+  uint32_t rep[BAL_K], jj[BAL_K];
+  bool ok[BAL_K];
+  double cfr[BAL_K], cfi[CPLX ? BAL_K : 1];
+  if constexpr (RUNS) {
+#pragma unroll 1
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      if (j < total) {
+        uint32_t r;
+        double cr = 0.0, ci = 0.0;
+        const int st = canon_runs<KIND, PH, LT>(B, tgt[j], r, cr, ci);
+        if (st > 0) {
+          tgt[j] = r;
+          tcr[j] = cr;
+          if (CPLX) tci[j] = ci;
+        } else if (st == 0) {
+          ent[j] = 0xff;
+        } else {
+          dq[atomicAdd(&ndef, 1u)] = (uint16_t)j;
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t nd = ndef;
+    for (uint32_t d = tid; d < nd; d += BAL_TPB) {
+      const uint32_t j = dq[d];
+      uint32_t r = 0;
+      double cr = 0.0, ci = 0.0;
+      const bool okb = (LT == 32) ? fast_canon32<KIND, PH>(B, tgt[j], r, cr, ci)
+                                  : fast_canon_w<KIND, PH>(B, tgt[j], r, cr, ci);
+      if (okb) {
+        tgt[j] = r;
+        tcr[j] = cr;
+        if (CPLX) tci[j] = ci;
+      } else {
+        ent[j] = 0xff;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      ok[q] = j < total && ent[j] != 0xff;
+      rep[q] = ok[q] ? tgt[j] : 0u;
+      cfr[q] = ok[q] ? tcr[j] : 0.0;
+      if (CPLX) cfi[q] = ok[q] ? tci[j] : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      ok[q] = j < total;
+      uint32_t t = ok[q] ? tgt[j] : 0u;
+      double cr = 0.0, ci = 0.0;
+      rep[q] = 0;
+      if (ok[q]) {
+        if (LT == 32) ok[q] = fast_canon32<KIND, PH>(B, t, rep[q], cr, ci);
+        else ok[q] = fast_canon<KIND, PH>(B, t, rep[q], cr, ci);
+      }
+      cfr[q] = cr;
+      if (CPLX) cfi[q] = ci;
+    }
+  }
+  region_lookup_many<BAL_K>(A, rep, ok, jj);
+  if constexpr (NV == 1) {
+    // one column: the gathers of a lane's targets issued together (as in
+    // k_matvec_bal), contributions to LDS; an absent target reads x[0] and
+    // is dropped
+    VT xg[BAL_K];
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) xg[q] = X[jj[q] != HASH_EMPTY ? jj[q] : 0u];
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      if (j < total) {
+        tgt[j] = jj[q];
+        const double v = jj[q] != HASH_EMPTY ? sV[0][ent[j]] : 0.0;
+        const ST cr = (ST)(v * cfr[q]);
+        ST pr, pi;
+        if (CPLX) {
+          const ST ci = (ST)(v * cfi[q]);
+          pr = cr * (ST)xg[q].x - ci * (ST)xg[q].y;
+          pi = cr * (ST)xg[q].y + ci * (ST)xg[q].x;
+        } else {
+          pr = cr * (ST)xg[q].x;
+          pi = cr * (ST)xg[q].y;
+        }
+        tcr[j] = (double)pr;
+        tci[j] = (double)pi;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < BAL_K; q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      if (j < total) {
+        tgt[j] = jj[q];
+        tcr[j] = cfr[q];
+        if (CPLX) tci[j] = cfi[q];
+      }
+    }
+  }
+  __syncthreads();
+  if constexpr (NV == 1) {
+    // phase 3, one column: row threads sum their contributions in order
+    if (own && M.y[0]) {
+      VT *yc = (VT *)M.y[0];
+      const uint32_t q0 = off, q1 = off + n;
+      ST dg = 0;
+      for (int f = 0; f < T.nfam; f++)
+        dg += (ST)(sD[0][f * 4 + 0] * rpop[tid][f * 4 + 0] + sD[0][f * 4 + 1] * rpop[tid][f * 4 + 1] +
+                   sD[0][f * 4 + 2] * rpop[tid][f * 4 + 2] + sD[0][f * 4 + 3] * rpop[tid][f * 4 + 3]);
+      ST orr = 0, oi = 0;
+      for (uint32_t q = q0; q < q1; q++) {
+        if (tgt[q] == HASH_EMPTY) continue;
+        orr += (ST)tcr[q];
+        oi += (ST)tci[q];
+      }
+      const VT xi = X[r0 + tid];
+      VT out;
+      out.x = dg * (ST)xi.x + orr * rinv[tid];
+      out.y = dg * (ST)xi.y + oi * rinv[tid];
+      yc[r0 + tid] = out;
+    }
+    return;
+  }
+  // phase 3: NV lanes per row; lane c sums its column in generator order with
+  // its operator's values, adds its diagonal and writes its output
+  for (uint32_t idx = tid; idx < nrow * NV; idx += BAL_TPB) {
+    const uint32_t row = idx / NV, c = idx % NV;
+    VT *yc = (VT *)M.y[c];
+    if (!yc) continue;
+    const uint32_t q0 = roff[row], q1 = q0 + rcnt[row];
+    ST dg = 0;
+    for (int f = 0; f < T.nfam; f++)
+      dg += (ST)(sD[c][f * 4 + 0] * rpop[row][f * 4 + 0] + sD[c][f * 4 + 1] * rpop[row][f * 4 + 1] +
+                 sD[c][f * 4 + 2] * rpop[row][f * 4 + 2] + sD[c][f * 4 + 3] * rpop[row][f * 4 + 3]);
+    ST orr = 0, oi = 0;
+    for (uint32_t q = q0; q < q1; q++) {
+      const uint32_t jq = tgt[q];
+      if (jq == HASH_EMPTY) continue;
+      const double v = sV[c][ent[q]];
+      const ST cr = (ST)(v * tcr[q]);
+      const VT xj = X[(size_t)jq * NV + c];
+      ST pr, pi;
+      if (CPLX) {
+        const ST ci = (ST)(v * tci[q]);
+        pr = cr * (ST)xj.x - ci * (ST)xj.y;
+        pi = cr * (ST)xj.y + ci * (ST)xj.x;
+      } else {
+        pr = cr * (ST)xj.x;
+        pi = cr * (ST)xj.y;
+      }
+      orr += pr;
+      oi += pi;
+    }
+    const VT xi = X[(size_t)(r0 + row) * NV + c];
+    VT out;
+    out.x = dg * (ST)xi.x + orr * rinv[row];
+    out.y = dg * (ST)xi.y + oi * rinv[row];
+    yc[r0 + row] = out;
+  }
+}
+
+template <int KIND, int PH, int LT, int NV>
+static void launch_mc(const FastArgs &A, const McArgs &M, int dtype, const void *X, hipStream_t st) {
+  if (dtype == EDK_C64)
+    k_matvec_bal_mc<KIND, PH, LT, double2, double, NV><<<A.nchunk, BAL_TPB, 0, st>>>(A, M, (const double2 *)X);
+  else
+    k_matvec_bal_mc<KIND, PH, LT, float2, float, NV><<<A.nchunk, BAL_TPB, 0, st>>>(A, M, (const float2 *)X);
+}
+
+template <int KIND, int PH, int NV>
+static void launch_mc_lt(const FastArgs &A, const McArgs &M, int dtype, const void *X, hipStream_t st) {
+  if (A.L == 32 && A.A == 1) launch_mc<KIND, PH, 32, NV>(A, M, dtype, X, st);
+  else if (A.A == 1 && A.L >= 16 && KIND != EDK_PROJECTED) launch_mc<KIND, PH, 1, NV>(A, M, dtype, X, st);
+  else launch_mc<KIND, PH, 0, NV>(A, M, dtype, X, st);
+}
+
+template <int KIND, int PH>
+static bool launch_mc_nv(const FastArgs &A, const McArgs &M, int dtype, const void *X, int nv, hipStream_t st) {
+  switch (nv) {
+    case 1: launch_mc_lt<KIND, PH, 1>(A, M, dtype, X, st); return true;
+    case 2: launch_mc_lt<KIND, PH, 2>(A, M, dtype, X, st); return true;
+    case 4: launch_mc_lt<KIND, PH, 4>(A, M, dtype, X, st); return true;
+    case 8: launch_mc_lt<KIND, PH, 8>(A, M, dtype, X, st); return true;
+    default: return false;
+  }
+}
+
+bool edk_fast_launch_mc(const FastArgs &A, const McArgs &M, int dtype, const void *X, int nv, hipStream_t st) {
+  if (!A.chunk) return false;
+  switch (A.kind) {
+    case EDK_TRANSLATION_PARITY:
+      if (A.kp == 1 && A.P == 1) return launch_mc_nv<EDK_TRANSLATION_PARITY, 0>(A, M, dtype, X, nv, st);
+      return launch_mc_nv<EDK_TRANSLATION_PARITY, 1>(A, M, dtype, X, nv, st);
+    case EDK_TRANSLATIONAL:
+      if (A.kp == 0) return launch_mc_nv<EDK_TRANSLATIONAL, 0>(A, M, dtype, X, nv, st);
+      if (2 * A.kp == A.len) return launch_mc_nv<EDK_TRANSLATIONAL, 1>(A, M, dtype, X, nv, st);
+      return launch_mc_nv<EDK_TRANSLATIONAL, 2>(A, M, dtype, X, nv, st);
+    case EDK_PROJECTED: return launch_mc_nv<EDK_PROJECTED, 0>(A, M, dtype, X, nv, st);
+    default: return false;
+  }
+}
+
+// X[i * nv + c] = xs[c][i]: nv plain vectors into one row-interleaved block
+template <typename VT>
+__global__ void __launch_bounds__(256) k_gather_cols(ColPtrs G, int nv, int64_t D, VT *__restrict__ X) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= D * nv) return;
+  const int64_t i = t / nv;
+  const int c = (int)(t - i * nv);
+  X[t] = ((const VT *)G.p[c])[i];
+}
+
+int edk_gather_cols(const ColPtrs &G, int nv, int64_t D, int dtype, void *X, hipStream_t st) {
+  for (int c = 0; c < nv; c++)
+    if (!G.p[c]) return edk_fail(EDK_ERR_ARG, "edk_gather_cols: null column");
+  const unsigned g = (unsigned)((D * nv + 255) / 256);
+  if (dtype == EDK_C64) k_gather_cols<double2><<<g, 256, 0, st>>>(G, nv, D, (double2 *)X);
+  else k_gather_cols<float2><<<g, 256, 0, st>>>(G, nv, D, (float2 *)X);
+  EDK_HIP(hipGetLastError());
+  return EDK_OK;
+}
+

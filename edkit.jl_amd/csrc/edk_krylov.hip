@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -1024,6 +1025,10 @@ struct edk_krylov {
   edk_krylov_diag diag;
   std::vector<double> accepted;
   KrylovWork K;
+  // partial reorthogonalisation (EDK_REORTH_PARTIAL): the omega estimates of
+  // the current Lanczos run, restarted with the basis; null: full
+  std::unique_ptr<PartialOrth> po;
+  double po_eps = 0;
 };
 
 extern "C" void edk_krylov_default_params(edk_krylov_params *p) {
@@ -1035,6 +1040,7 @@ extern "C" void edk_krylov_default_params(edk_krylov_params *p) {
   p->bisect_iters = 12;
   p->extend_basis = 1;
   p->normalize_output = 0;
+  p->reorth = EDK_REORTH_FULL;
 }
 
 // _build_reduced! (TimeEvolution.jl:311-323)
@@ -1133,7 +1139,7 @@ static double max_valid(const edk_krylov *c, double tau) {
 static int lanczos_build(edk_krylov *c, int m_target) {
   m_target = std::min(m_target, c->m_max);
   int r = lanczos_extend(c->H, c->K, c->V, c->N, c->w, c->alpha.data(), c->beta.data(), &c->m, m_target,
-                         &c->diag.matvecs, c->m_max + 1);
+                         &c->diag.matvecs, c->m_max + 1, c->po.get());
   return r < 0 ? -r : EDK_OK;
 }
 
@@ -1163,6 +1169,7 @@ static int restart(edk_krylov *c, double tau) {
   std::fill(c->alpha.begin(), c->alpha.end(), 0.0);
   std::fill(c->beta.begin(), c->beta.end(), 0.0);
   c->m = 0;
+  if (c->po) c->po = std::make_unique<PartialOrth>(c->po_eps);
   if (int rc = c->K.scale(c->V, c->psi_anchor, 1.0 / nrm)) return rc;
   if (int rc = lanczos_build(c, c->m_init)) return rc;
   if (int rc = build_reduced(c)) return rc;
@@ -1226,6 +1233,8 @@ int edk_krylov_create_on(edk_op *op, edk_ctx *ctx, int dtype, const void *psi0, 
   if (prm.bisect_iters < 0) return edk_fail(EDK_ERR_ARG, "bisect_iters must be >= 0");
   if (prm.extend_step < 1) return edk_fail(EDK_ERR_ARG, "extend_step must be >= 1");
   if (prm.m_max > EDK_KRYLOV_MAXM) return edk_fail(EDK_ERR_UNSUPPORTED, "m_max above 1024");
+  if (prm.reorth != EDK_REORTH_FULL && prm.reorth != EDK_REORTH_PARTIAL)
+    return edk_fail(EDK_ERR_ARG, "reorth must be EDK_REORTH_FULL or EDK_REORTH_PARTIAL");
   EDK_HIP(hipSetDevice(ctx->device));
   edk_krylov *c = new edk_krylov();
   c->H = op;
@@ -1240,6 +1249,8 @@ int edk_krylov_create_on(edk_op *op, edk_ctx *ctx, int dtype, const void *psi0, 
   c->bisect_iters = prm.bisect_iters;
   c->extend_basis = prm.extend_basis;
   c->normalize_output = prm.normalize_output;
+  c->po_eps = dtype == EDK_C64 ? 2.220446049250313e-16 : 1.1920928955078125e-07;
+  if (prm.reorth == EDK_REORTH_PARTIAL) c->po = std::make_unique<PartialOrth>(c->po_eps);
   memset(&c->diag, 0, sizeof(c->diag));
   c->alpha.assign(c->m_max, 0.0);
   c->beta.assign(c->m_max, 0.0);

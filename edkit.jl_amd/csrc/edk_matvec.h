@@ -75,6 +75,23 @@ int edk_fast_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant,
 bool edk_matvec_prepare(edk_op *op);
 // one-time setup of the generic tile kernel (allocates; run before capture)
 bool edk_generic_setup(edk_op *op);
+// Batched products of operators sharing a basis and a two-site family
+// structure (different values): edk_fast_signature gives the structure's hash
+// (false: no specialised kernel for op); edk_fast_matvec_mc sets
+// ys[c] = H_c xs[c] for n <= 8 operators through one launch, with the
+// row-interleaved scratch block Xblk of D x 8 elements.
+bool edk_fast_signature(edk_op *op, uint64_t *sig);
+int edk_fast_matvec_mc(edk_op *const *ops, int n, int dtype, const void *const *xs, void *const *ys, void *Xblk,
+                       hipStream_t st);
+// Matvec interception for the calling thread (the server's batcher): when
+// set, edk_matvec_launch hands single-vector, non-accumulating products of
+// ComplexF64/ComplexF32 vectors to it; it returns EDK_OK when it took the
+// product, or a negative value to run the product directly.
+struct edk_mv_hook {
+  virtual int launch(edk_op *op, int dtype, void *y, const void *x, hipStream_t st) = 0;
+  virtual ~edk_mv_hook() = default;
+};
+void edk_set_mv_hook(edk_mv_hook *h);
 // y = (accumulate ? y : 0) + H x on stream st (no allocation, no synchronisation)
 int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);
 inline int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate) {

@@ -835,9 +835,16 @@ extern "C" int edk_operator_count_offdiag(edk_op *op, int64_t *nnz) {
   return EDK_OK;
 }
 
+static thread_local edk_mv_hook *tl_mv_hook = nullptr;
+void edk_set_mv_hook(edk_mv_hook *h) { tl_mv_hook = h; }
+
 int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st) {
   MatvecArgs a = op->args();
   if (a.D == 0) return EDK_OK;
+  if (tl_mv_hook && nb == 1 && !accumulate) {
+    const int rc = tl_mv_hook->launch(op, dtype, y, x, st);
+    if (rc >= 0) return rc;
+  }
   // several right-hand sides with the assembled matrix cached (sparse!)
   if (nb > 1 && edk_matvec_csr(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
@@ -859,6 +866,28 @@ extern "C" int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t
   if (x == y) return edk_fail(EDK_ERR_ARG, "edk_matvec: x and y must not alias");
   EDK_HIP(hipSetDevice(op->ctx->device));
   return edk_matvec_launch(op, dtype, y, x, nb, accumulate);
+}
+
+extern "C" int edk_matvec_multi(edk_op *const *ops, int n, int dtype, void *const *ys, const void *const *xs) {
+  if (!ops || !ys || !xs || n < 1 || n > 8) return edk_fail(EDK_ERR_ARG, "edk_matvec_multi: 1..8 operators");
+  if (dtype != EDK_C64 && dtype != EDK_C32) return edk_fail(EDK_ERR_ARG, "edk_matvec_multi: bad dtype");
+  for (int c = 0; c < n; c++) {
+    if (!ops[c] || !ys[c] || !xs[c]) return edk_fail(EDK_ERR_ARG, "edk_matvec_multi: null argument");
+    if (ops[c]->basis != ops[0]->basis) return edk_fail(EDK_ERR_ARG, "edk_matvec_multi: operators on different bases");
+    for (int q = 0; q < n; q++)
+      if (xs[q] == ys[c]) return edk_fail(EDK_ERR_ARG, "edk_matvec_multi: an input aliases an output");
+  }
+  edk_ctx *ctx = ops[0]->ctx;
+  EDK_HIP(hipSetDevice(ctx->device));
+  const int64_t D = ops[0]->basis->D;
+  if (D == 0) return EDK_OK;
+  void *X = nullptr;
+  const size_t bytes = (size_t)D * 8 * (dtype == EDK_C64 ? 16 : 8);
+  if (edk_dalloc(ctx, &X, bytes) != hipSuccess) return edk_fail(EDK_ERR_OOM, "edk_matvec_multi block");
+  int rc = edk_fast_matvec_mc(ops, n, dtype, xs, ys, X, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc) rc = edk_fail(EDK_ERR_HIP, "edk_matvec_multi");
+  edk_dfree(ctx, X);
+  return rc;
 }
 
 extern "C" int edk_matvec_host(edk_op *op, int dtype, void *yh, const void *xh, int64_t nb, int accumulate) {

@@ -23,7 +23,7 @@ from .lindblad import (DensityMatrix, Lindblad, LindbladArnoldiCache, Liouvillia
                        densitymatrix, lindblad, lindblad_timeevolve, lindblad_timeevolve_)
 from .krylov import KrylovEvolutionCache, lanczos_ground, timeevolve, timeevolve_  # noqa: F401
 from .matvec import (Array, addto_, apply, clear_sparse_cache_, device_operator, matvec_into, mul,  # noqa: F401
-                     mul_, sparse, sparse_)
+                     mul_, mul_multi, sparse, sparse_)
 from .operator import Operator, find_base, operator, trans_inv_operator  # noqa: F401
 from .serve import Server  # noqa: F401
 from .spin import spin  # noqa: F401

@@ -28,12 +28,12 @@ SYMBOLS = [
     "edk_basis_build", "edk_basis_upload", "edk_basis_dim", "edk_basis_order",
     "edk_basis_download", "edk_basis_index", "edk_basis_destroy",
     "edk_operator_create", "edk_operator_destroy", "edk_operator_count_offdiag",
-    "edk_matvec", "edk_matvec_host",
+    "edk_matvec", "edk_matvec_host", "edk_matvec_multi",
     "edk_operator_assemble", "edk_operator_download_csr", "edk_operator_release_csr",
     "edk_operator_set_variant", "edk_operator_autotune",
     "edk_batch_create", "edk_batch_launch", "edk_batch_destroy",
-    "edk_server_create", "edk_server_submit", "edk_server_wait", "edk_server_clock",
-    "edk_server_stats", "edk_server_destroy",
+    "edk_server_default_params", "edk_server_create", "edk_server_create_ex", "edk_server_submit",
+    "edk_server_wait", "edk_server_clock", "edk_server_stats", "edk_server_batch_stats", "edk_server_destroy",
     "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2", "edk_vec_dot_noalloc",
     "edk_lanczos_ground",
     "edk_lanczos_ground_ex",
@@ -87,6 +87,10 @@ class JobResult(C.Structure):
                 ("t_end", C.c_double), ("message", C.c_char * 160)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("batch", C.c_int32), ("batch_wait_us", C.c_double), ("affinity", C.c_int32)]
+
+
 class ServerInfo(C.Structure):
     _fields_ = [("completed", C.c_int64), ("queued", C.c_int64), ("running", C.c_int64),
                 ("peak_running", C.c_int64), ("inflight_bytes", C.c_int64),
@@ -96,7 +100,7 @@ class ServerInfo(C.Structure):
 class KrylovParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("m_init", C.c_int32), ("m_max", C.c_int32),
                 ("extend_step", C.c_int32), ("nsample", C.c_int32), ("bisect_iters", C.c_int32),
-                ("extend_basis", C.c_int32), ("normalize_output", C.c_int32)]
+                ("extend_basis", C.c_int32), ("normalize_output", C.c_int32), ("reorth", C.c_int32)]
 
 
 class KrylovDiag(C.Structure):
@@ -159,6 +163,7 @@ def lib():
         "edk_operator_count_offdiag": (I32, [P, C.POINTER(C.c_int64)]),
         "edk_matvec": (I32, [P, I32, P, P, I64, I32]),
         "edk_matvec_host": (I32, [P, I32, P, P, I64, I32]),
+        "edk_matvec_multi": (I32, [P, I32, I32, P, P]),
         "edk_operator_assemble": (I32, [P, C.POINTER(C.c_int64)]),
         "edk_operator_download_csr": (I32, [P, P, P, P]),
         "edk_operator_release_csr": (I32, [P]),
@@ -168,6 +173,9 @@ def lib():
         "edk_batch_launch": (I32, [P]),
         "edk_batch_destroy": (I32, [P]),
         "edk_server_create": (I32, [I32, I32, SZ, PP]),
+        "edk_server_default_params": (None, [C.POINTER(ServerParams)]),
+        "edk_server_create_ex": (I32, [I32, I32, SZ, C.POINTER(ServerParams), PP]),
+        "edk_server_batch_stats": (I32, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(D)]),
         "edk_server_submit": (I32, [P, C.POINTER(Job), C.POINTER(C.c_int64)]),
         "edk_server_wait": (I32, [P, I64, D, C.POINTER(JobResult), P]),
         "edk_server_clock": (D, [P]),

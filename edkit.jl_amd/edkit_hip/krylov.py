@@ -51,14 +51,17 @@ class KrylovEvolutionCache:
 
     def __init__(self, H, psi0, tol=1e-10, m_init=30, m_max=60, extend_step=10, nsample=13,
                  bisect_iters=12, reuse_basis=True, extend_basis=True, normalize_output=False,
-                 dtype=np.complex128):
+                 dtype=np.complex128, reorth="full"):
         self.H = H
         self.op = device_operator(H)
         self.B = H.B
         self.psi0 = _to_device(psi0, self.B, dtype)
         self.dtype = self.psi0.dtype
+        if reorth not in REORTH_CODES:
+            raise ValueError("reorth must be 'full' or 'partial'")
         p = KrylovParams(float(tol), int(m_init), int(m_max), int(extend_step), int(nsample),
-                         int(bisect_iters), int(bool(extend_basis)), int(bool(normalize_output)))
+                         int(bisect_iters), int(bool(extend_basis)), int(bool(normalize_output)),
+                         REORTH_CODES[reorth])
         h = C.c_void_p()
         check(lib().edk_krylov_create(self.op.h, dtype_code(self.dtype), dev_ptr(self.psi0),
                                       C.byref(p), C.byref(h)))

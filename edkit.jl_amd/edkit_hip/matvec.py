@@ -216,3 +216,27 @@ def clear_sparse_cache_(H=None):
 def mul(H, v):
     """EDKit.mul(H, v) (Operator.jl:340-370): same result as H * v."""
     return apply(H, v)
+
+
+def mul_multi(Hs, xs):
+    """[H_c @ x_c for c] for up to 8 operators on one basis that share their
+    two-site family structure (e.g. XXZ chains of different Delta): one
+    launch, every target canonicalised and looked up once for all columns
+    (edk_matvec_multi).  xs: DeviceArrays or host vectors of one dtype;
+    returns DeviceArrays, or host arrays for host inputs."""
+    if not 1 <= len(Hs) == len(xs) <= 8:
+        raise ValueError("mul_multi: 1..8 operators, one vector each")
+    B = Hs[0].B
+    host = not isinstance(xs[0], DeviceArray)
+    dx = [DeviceArray.from_host(np.ascontiguousarray(x), ctx=B.ctx) if host else x for x in xs]
+    dt = dx[0].dtype
+    if any(x.dtype != dt or x.shape != (B.D,) for x in dx):
+        raise ValueError("mul_multi: vectors of one dtype and the basis dimension")
+    ops = [device_operator(H) for H in Hs]
+    dy = [DeviceArray((B.D,), dt, ctx=B.ctx, zero=False) for _ in Hs]
+    n = len(Hs)
+    P = C.c_void_p * n
+    check(lib().edk_matvec_multi(P(*[o.h.value for o in ops]), n, dtype_code(dt), P(*[dev_ptr(y) for y in dy]),
+                                 P(*[dev_ptr(x) for x in dx])))
+    return [y.to_host() for y in dy] if host else dy
+

@@ -1,25 +1,33 @@
 """Serving loop: Lanczos ground-state and Krylov time-evolution jobs on one
 device, admitted by worker slot and HBM footprint (libedkit_hip's
 edk_server_*).  Jobs of any sector and Hamiltonian run side by side on their
-own streams; each result is the same alone and in a mix.
+own streams; jobs sharing a sector and term structure share their matvecs,
+one block launch per Krylov step; each result is the same alone and in a mix.
 """
 import ctypes as C
 
 import numpy as np
 
-from ._lib import (EDK_JOB_EVOLVE, EDK_JOB_GROUND, REORTH_CODES, Job, JobResult, ServerInfo, check,
-                   lib)
+from ._lib import (EDK_JOB_EVOLVE, EDK_JOB_GROUND, REORTH_CODES, Job, JobResult, ServerInfo, ServerParams,
+                   check, lib)
 from .device import dtype_code
 from .matvec import device_operator
 
 
 class Server:
     """edk_server handle: `workers` threads (one stream each), admission
-    budget `hbm_budget` bytes (0: 80 % of free device memory)."""
+    budget `hbm_budget` bytes (0: 80 % of free device memory).  batch: the
+    matvecs of jobs sharing a sector and term structure run as one block
+    launch per Krylov step (up to 8 jobs); batch_wait_us: longest wait for a
+    group's other members; affinity: admit queued jobs that join a running
+    group first."""
 
-    def __init__(self, workers=8, hbm_budget=0, device=0):
+    def __init__(self, workers=8, hbm_budget=0, device=0, batch=True, batch_wait_us=250.0, affinity=True):
+        p = ServerParams()
+        lib().edk_server_default_params(C.byref(p))
+        p.batch, p.batch_wait_us, p.affinity = int(batch), float(batch_wait_us), int(affinity)
         h = C.c_void_p()
-        check(lib().edk_server_create(int(device), int(workers), int(hbm_budget), C.byref(h)))
+        check(lib().edk_server_create_ex(int(device), int(workers), int(hbm_budget), C.byref(p), C.byref(h)))
         self.h = h
         self._pending = {}
 
@@ -54,10 +62,14 @@ class Server:
                             refine=int(refine), want_vector=int(want_vector), reorth=reorth)
 
     def submit_evolve(self, H, t, psi0=None, psi0_index=0, tol=1e-10, m_init=30, m_max=60,
-                      dtype=np.complex128, want_vector=False):
-        """exp(-iHt) psi0 through KrylovEvolutionCache semantics; returns a ticket."""
+                      dtype=np.complex128, want_vector=False, reorth="full"):
+        """exp(-iHt) psi0 through KrylovEvolutionCache semantics; returns a
+        ticket.  reorth "partial" keeps the Krylov basis semi-orthogonal
+        (Simon's omega recurrence) instead of the reference's full sweeps."""
+        if reorth not in REORTH_CODES:
+            raise ValueError("reorth must be 'full' or 'partial'")
         return self._submit(H, EDK_JOB_EVOLVE, dtype, psi0, psi0_index, t=t, tol=tol,
-                            m_init=m_init, m_max=m_max, want_vector=int(want_vector))
+                            m_init=m_init, m_max=m_max, want_vector=int(want_vector), reorth=reorth)
 
     def wait(self, ticket, timeout=-1.0):
         """Block until the job is done; returns (result dict, vector or None)."""
@@ -78,6 +90,14 @@ class Server:
         st = ServerInfo()
         check(lib().edk_server_stats(self.h, C.byref(st)))
         return {f: getattr(st, f) for f, _ in ServerInfo._fields_}
+
+    def batch_stats(self, weighted=False):
+        """(block launches, columns carried) of the batched matvecs so far;
+        weighted=True adds the columns per launch weighted by the sector
+        dimension."""
+        a, b, w = C.c_int64(), C.c_int64(), C.c_double()
+        check(lib().edk_server_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(w)))
+        return (a.value, b.value, w.value) if weighted else (a.value, b.value)
 
     def close(self):
         if getattr(self, "h", None):

@@ -159,6 +159,15 @@ int edk_matvec(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
 /* The literal host-array mul!(y, H, x): copies in, applies, copies out. */
 int edk_matvec_host(edk_op *op, int dtype, void *y_host, const void *x_host,
                     int64_t nb, int accumulate);
+/* ys[c] = H_c xs[c] (zero-filled targets, as H*x) for n <= 8 operators on one
+ * basis that share a two-site family structure and differ in their values
+ * (e.g. XXZ chains of different Delta, J1-J2 rings of different J2): one
+ * launch on ops[0]'s context stream, every target canonicalised and looked up
+ * once for all n columns; the same bits per column as a launch with that
+ * column alone.  The loop `for c: mul!(ys[c], H_c, xs[c])` over Operator.jl:
+ * 309-315 with one traversal.  EDK_ERR_UNSUPPORTED when the operators have no
+ * shared specialised kernel (then call edk_matvec per operator). */
+int edk_matvec_multi(edk_op *const *ops, int n, int dtype, void *const *ys, const void *const *xs);
 
 /* ---- explicit sector matrix --------------------------------------------------- */
 /* sparse(opt) / Array(opt) (Operator.jl:171-217) and the sparse!(opt) cache
@@ -287,6 +296,7 @@ typedef struct {
   int32_t bisect_iters;/* 12 */
   int32_t extend_basis;     /* 1 */
   int32_t normalize_output; /* 0 */
+  int32_t reorth;           /* EDK_REORTH_FULL (the reference's _lanczos_build!) or EDK_REORTH_PARTIAL */
 } edk_krylov_params;
 
 typedef struct {
@@ -388,11 +398,21 @@ int edk_lindblad_destroy(edk_lindblad *c);
 /* No counterpart in the reference.  A server runs Lanczos ground-state and
  * Krylov time-evolution jobs (any sector and operator on its device) on
  * `workers` threads, each with its own stream, so jobs overlap on the GPU.
- * Jobs are admitted first-in first-out as a worker frees up and the job's
- * device footprint fits beside the running ones within hbm_budget bytes
- * (0: 80 % of the free memory at creation).  The start vector is copied at
- * submit; the job's operator and basis must stay alive until it finishes.
- * A job's result is independent of what else runs (fixed-order reductions). */
+ * A job is admitted as a worker frees up and its device footprint fits beside
+ * the running ones within hbm_budget bytes (0: 80 % of the free memory at
+ * creation).  Jobs whose operators share a basis and a two-site family
+ * structure (one sector, different coefficients) form a group; with batching
+ * on, their matvecs are launched together step by step, one block launch for
+ * up to 8 members, and admission prefers queued jobs that join a running
+ * group (continuous batching).  The start vector is copied at submit; the
+ * job's operator and basis must stay alive until it finishes.  A job's result
+ * is independent of what else runs: the same bits alone or in any mix. */
+typedef struct {
+  int32_t batch;        /* 1: batch the matvecs of group members */
+  double batch_wait_us; /* longest wait for a group's other members before a partial block launches */
+  int32_t affinity;     /* 1: admit queued jobs joining a running group before the queue head */
+} edk_server_params;
+void edk_server_default_params(edk_server_params *p); /* batch 1, 250 us, affinity 1 */
 #define EDK_JOB_GROUND 0 /* edk_lanczos_ground on the start vector */
 #define EDK_JOB_EVOLVE 1 /* KrylovEvolutionCache(H, psi0) then timeevolve!(cache, t) */
 typedef struct {
@@ -409,7 +429,7 @@ typedef struct {
   int32_t m_init;      /* evolve: Krylov dimensions (<= 0: 30 and 60) */
   int32_t m_max;
   int32_t want_vector; /* 1: keep the result vector (ground state or psi(t)) */
-  int32_t reorth;      /* ground: EDK_REORTH_FULL or EDK_REORTH_PARTIAL */
+  int32_t reorth;      /* EDK_REORTH_FULL or EDK_REORTH_PARTIAL (ground state and evolution) */
 } edk_job;
 typedef struct {
   int32_t status;      /* EDK_OK or the error code of the job */
@@ -425,7 +445,9 @@ typedef struct {
   int64_t inflight_bytes, peak_inflight_bytes, budget_bytes;
 } edk_server_info;
 typedef struct edk_server edk_server;
-int edk_server_create(int device, int workers, size_t hbm_budget, edk_server **out);
+int edk_server_create(int device, int workers, size_t hbm_budget, edk_server **out); /* default params */
+int edk_server_create_ex(int device, int workers, size_t hbm_budget, const edk_server_params *p,
+                         edk_server **out);
 int edk_server_submit(edk_server *s, const edk_job *job, int64_t *ticket);
 /* Wait for a job (timeout_s < 0: no limit; EDK_ERR_STATE on timeout) and
  * release its ticket; vec_host (D elements) receives the kept vector. */
@@ -433,6 +455,9 @@ int edk_server_wait(edk_server *s, int64_t ticket, double timeout_s, edk_job_res
                     void *vec_host);
 double edk_server_clock(edk_server *s); /* the server's clock, seconds */
 int edk_server_stats(edk_server *s, edk_server_info *st);
+/* block launches of batched matvecs so far, the columns they carried, and
+ * the columns per launch weighted by the sector dimension */
+int edk_server_batch_stats(edk_server *s, int64_t *launches, int64_t *columns, double *dim_weighted);
 int edk_server_destroy(edk_server *s); /* queued jobs are dropped, running ones finish */
 
 #ifdef __cplusplus

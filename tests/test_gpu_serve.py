@@ -1,8 +1,10 @@
 """GPU tests of the serving loop: every job's result is the same alone (one
-worker, one job at a time), in a mix (several workers, all jobs queued at
-once) and through the direct entry points -- bit for bit, since every
-reduction has a fixed order -- and matches dense diagonalisation / the oracle
-where the sector is small; admission by HBM footprint and error reporting."""
+worker, one job at a time) and in a mix (several workers, all jobs queued at
+once) bit for bit -- every reduction has a fixed order and a batched column's
+arithmetic does not depend on the other columns -- agrees with the direct
+entry points (which use the single-vector kernel: energies within 1e-11,
+states within 1e-9) and matches dense diagonalisation / the oracle where the
+sector is small; admission by HBM footprint and error reporting."""
 import numpy as np
 import pytest
 
@@ -58,15 +60,20 @@ def test_results_same_alone_and_in_mix(gpu):
         assert np.array_equal(va, vm), kind
         for f in ("E0", "resid", "steps", "matvecs", "overlap_re", "overlap_im", "norm"):
             assert ra[f] == rm[f], (kind, f)
-    # the direct entry points give the same numbers
+    # the direct entry points agree (their single-vector kernel rounds the
+    # products in another order than the server's block launch)
     for (kind, H, kw), (ra, va) in zip(jobs, alone):
         if kind != "ground":
             continue
         E, r, steps, vec = edk.lanczos_ground(H, kw["psi0"], m=kw["m"], tol=kw["tol"],
                                               refine=kw.get("refine", False), want_vec=True,
                                               reorth=kw.get("reorth", "full"))
-        assert (E, r, steps) == (ra["E0"], ra["resid"], ra["steps"])
-        assert np.array_equal(vec.to_host(), va)
+        assert abs(E - ra["E0"]) < 1e-11 and r <= kw["tol"] and ra["resid"] <= kw["tol"]
+        assert abs(steps - ra["steps"]) <= 2
+        vd = vec.to_host().astype(np.complex128)
+        vs = va.astype(np.complex128)
+        ov = abs(np.vdot(vd, vs)) / (np.linalg.norm(vd) * np.linalg.norm(vs))
+        assert ov > 1 - 1e-9, ov
 
 
 def test_served_results_match_dense(gpu):
