@@ -86,6 +86,15 @@ struct FastArgs {
   const uint32_t *chunk; // balanced schedule: first row of each chunk (nchunk + 1 entries), or null
   uint32_t nchunk;
   uint32_t xcdg;         // balanced schedule: consecutive chunks given to one XCD (0: plain order)
+  // rank directory (null when it does not apply): every representative is a
+  // word with the bits of fix1 set, those of fix0 clear and rn of the rbits
+  // bits in dmask (from bit rlo up) set; its colex rank among such words
+  // indexes granules of 96 ranks {representatives below the granule, bitmap}
+  const uint4 *gran;
+  uint32_t ngran;
+  const uint32_t *rtab;   // 3-bit group table of the colex rank (RK_NT entries)
+  uint32_t fix1, fix0, dmask;
+  int rlo, rbits, rn, rng;  // domain: lowest bit, width, set bits; groups of 3 bits
 };
 
 __device__ __forceinline__ uint32_t rotr32(uint32_t s, int sh, int L, uint32_t mask) {
@@ -726,6 +735,87 @@ __device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint
 }
 
 // ---------------------------------------------------------------------------
+// Rank directory.  Representatives with a fixed number of set bits are
+// numbered by their colex rank, which for words of one popcount is their
+// numeric order: with p_1 < p_2 < ... the positions of the set domain bits,
+// rank = sum_i C(p_i, i).  A granule covers 96 consecutive ranks: the number
+// of representatives of lower rank and a 96-bit membership bitmap, so a
+// lookup is one 16-byte load at rank / 96.  The representatives cluster in
+// rank (L = 32, N = 16: 9.4 M of them fill 13 % of the granules below the
+// largest rank), so the directory's hot part is a few MB against the 56 MB of
+// a key table with a bucket directory.
+//
+// Ranks are summed over 3-bit groups of the domain word from a table
+// T[g][c][v] = sum over the set bits b of v, the j-th of them (j = 0, 1, ..),
+// of C(3 g + b, c + j + 1), where c counts the set bits below the group.
+// ---------------------------------------------------------------------------
+#define RK_NCNT 17                   // c = 0 .. 16 (at most 16 domain bits set)
+#define RK_NGRP 11                   // 3-bit groups of a 32-bit domain
+#define RK_NT (RK_NGRP * RK_NCNT * 8)
+#define RK_GRAN 96u                  // ranks per granule
+
+// colex rank of the domain word w (ng groups)
+__device__ __forceinline__ uint32_t rank_full(const uint32_t *T, uint32_t w, int ng) {
+  uint32_t r = 0, c = 0;
+  for (int g = 0; g < ng; g++) {
+    const uint32_t v = (w >> (3 * g)) & 7u;
+    r += T[(g * RK_NCNT + c) * 8 + v];
+    c += __popc(v);
+  }
+  return r;
+}
+
+// C(n, k) for n < 32, k <= 17 from the group table (a single set bit)
+__device__ __forceinline__ uint32_t rank_binom(const uint32_t *T, int n, int k) {
+  const int g = (n * 11) >> 5;  // n / 3 for n < 32
+  const int idx = k > 0 ? ((g * RK_NCNT + k - 1) * 8 + (1 << (n - 3 * g))) : 0;
+  return k > 0 ? T[idx] : 1u;
+}
+
+// colex terms of the set bits of a window v of at most 3 bits at domain
+// position a, with c domain bits set below it
+__device__ __forceinline__ uint32_t rank_window(const uint32_t *T, uint32_t v, int a, int c) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    const bool on = (v >> b) & 1u;
+    c += on;
+    r += on ? rank_binom(T, a + b, c) : 0u;
+  }
+  return r;
+}
+
+// a word is in the rank domain: fixed bits as required, rn domain bits set
+__device__ __forceinline__ bool rank_domain(const FastArgs &A, uint32_t r) {
+  return (r & A.fix1) == A.fix1 && (r & A.fix0) == 0 && __popc(r & A.dmask) == A.rn;
+}
+
+// positions in the table of K ranks (HASH_EMPTY: not a representative); the
+// K granule loads issue together and every result is formed without a branch
+template <int K>
+__device__ __forceinline__ void rank_lookup_many(const FastArgs &A, const uint32_t (&rk)[K], const bool (&ok)[K],
+                                                 uint32_t (&jj)[K]) {
+  uint4 g[K];
+  bool in[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const uint32_t gi = rk[q] / RK_GRAN;
+    in[q] = ok[q] && gi < A.ngran;
+    g[q] = A.gran[in[q] ? gi : 0u];
+  }
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const uint32_t o = rk[q] % RK_GRAN;
+    const uint32_t wsel = o < 32u ? g[q].y : (o < 64u ? g[q].z : g[q].w);
+    const uint32_t bit = o & 31u;
+    const uint32_t below = (o >= 32u ? __popc(g[q].y) : 0u) + (o >= 64u ? __popc(g[q].z) : 0u) +
+                           __popc(wsel & ((1u << bit) - 1u));
+    const bool hit = in[q] && ((wsel >> bit) & 1u);
+    jj[q] = (g[q].x + below) | (hit ? 0u : HASH_EMPTY);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // balanced kernel: one block per chunk of whole rows holding at most
 // BAL_CAP targets.  Phase 1: the row threads write their targets (and entry
 // numbers) into LDS in generator order.  Phase 2: every thread takes BAL_K
@@ -801,6 +891,8 @@ struct FastState {
   uint2 *d_region = nullptr;
   uint16_t *d_fine = nullptr;
   uint32_t *d_chunk = nullptr;
+  uint4 *d_gran = nullptr;      // rank directory granules
+  uint32_t *d_rtab = nullptr;   // its rank table
   bool cplx = false;
   int tuned[2][9] = {};  // schedule per (dtype, block width 1..8), EDK_VARIANT_*
 };
@@ -808,6 +900,9 @@ struct FastState {
 
 // block-of-vectors launch (edk_fast_blk.hip); nv in {2, 4, 8}
 bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, int nv, int acc, hipStream_t st);
+// one-vector launch with the rank directory (edk_fast_rk.hip); false when it
+// does not apply (no directory, complex phases, not a run-length canonicaliser)
+bool edk_fast_launch_rk(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st);
 // multi-coefficient block launch (edk_fast_blk.hip): nv in {1, 2, 4, 8}
 // columns of the row-interleaved block X (D x nv), column c with the values of
 // M.ctab[c], written to M.y[c]; A is any member's arguments (shared structure)

@@ -401,6 +401,32 @@ __global__ void k_dir32(const uint32_t *keys, int64_t D, int dshift, uint32_t nd
   dir[b] = (uint32_t)lo;
 }
 
+// colex ranks of the keys (sorted, so the ranks increase)
+__global__ void k_rank_keys(const uint32_t *keys, uint32_t D, const uint32_t *T, uint32_t dmask, int rlo, int ng,
+                            uint32_t *rk) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D) rk[i] = rank_full(T, (keys[i] & dmask) >> rlo, ng);
+}
+
+// granule g: representatives of rank below 96 g, membership of 96 g .. 96 g + 95
+__global__ void k_gran_build(const uint32_t *rk, uint32_t D, uint32_t ngran, uint4 *gran) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngran) return;
+  const uint64_t r0 = (uint64_t)g * RK_GRAN;
+  uint32_t lo = 0, hi = D;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint64_t)rk[mid] < r0) lo = mid + 1;
+    else hi = mid;
+  }
+  uint32_t w[3] = {0u, 0u, 0u};
+  for (uint32_t i = lo; i < D && (uint64_t)rk[i] < r0 + RK_GRAN; i++) {
+    const uint32_t o = (uint32_t)(rk[i] - r0);
+    w[o >> 5] |= 1u << (o & 31u);
+  }
+  gran[g] = make_uint4(lo, w[0], w[1], w[2]);
+}
+
 static FastState *fast_state(edk_op *op) {
   if (!op->fast) op->fast = new FastState();
   return (FastState *)op->fast;
@@ -416,6 +442,8 @@ void edk_fast_release(edk_op *op) {
   (void)hipFree(F->d_region);
   (void)hipFree(F->d_fine);
   (void)hipFree(F->d_chunk);
+  (void)hipFree(F->d_gran);
+  (void)hipFree(F->d_rtab);
   delete F;
   op->fast = nullptr;
 }
@@ -505,6 +533,86 @@ static bool fast_setup(edk_op *op) {
   return ok;
 }
 
+// Rank directory for the run-length canonicalisers (unit cell 1, 16 <= L <=
+// 32, translation kinds): applies when every representative has one
+// popcount; the domain leaves out bit 0 when every key has it set and the top
+// bit when every key has it clear (true of smallest rotations).  Returns
+// false only on a device error; the directory is left out when it does not
+// apply.
+static bool rank_setup(edk_op *op, FastState *S, const std::vector<uint32_t> &keys) {
+  const edk_basis *b = op->basis;
+  FastArgs &A = S->args;
+  A.gran = nullptr;
+  if (b->A != 1 || b->L < 16 || b->L > 32 || (b->kind != EDK_TRANSLATION_PARITY && b->kind != EDK_TRANSLATIONAL))
+    return true;
+  const int L = b->L;
+  const uint32_t top = 1u << (L - 1);
+  const int pc = __builtin_popcount(keys[0]);
+  bool all1 = true, all0 = true;
+  for (uint32_t k : keys) {
+    if (__builtin_popcount(k) != pc) return true;
+    all1 = all1 && (k & 1u);
+    all0 = all0 && !(k & top);
+  }
+  const int rlo = all1 ? 1 : 0, rbits = L - rlo - (all0 ? 1 : 0), rn = pc - rlo;
+  if (rn < 0 || rn >= RK_NCNT || rbits < 1) return true;
+  // Pascal's triangle to n = 35, then the 3-bit group table
+  uint64_t Cb[36][36] = {};
+  for (int n = 0; n < 36; n++) {
+    Cb[n][0] = 1;
+    for (int k = 1; k <= n; k++) Cb[n][k] = Cb[n - 1][k - 1] + (k < n ? Cb[n - 1][k] : 0);
+  }
+  if (Cb[rbits][rn] >= 0xffffffffull - RK_GRAN) return true;
+  std::vector<uint32_t> T(RK_NT, 0u);
+  for (int g = 0; g < RK_NGRP; g++)
+    for (int c = 0; c < RK_NCNT; c++)
+      for (int v = 0; v < 8; v++) {
+        uint64_t t = 0;
+        for (int b = 0, j = 0; b < 3; b++)
+          if ((v >> b) & 1) {
+            const int n = 3 * g + b, k = c + j + 1;
+            t += (k <= n && n < 36) ? Cb[n][k] : 0;
+            j++;
+          }
+        T[(g * RK_NCNT + c) * 8 + v] = (uint32_t)t;
+      }
+  A.rlo = rlo;
+  A.rbits = rbits;
+  A.rn = rn;
+  A.rng = (rbits + 2) / 3;
+  A.fix1 = all1 ? 1u : 0u;
+  A.fix0 = all0 ? top : 0u;
+  A.dmask = (uint32_t)((((uint64_t)1 << rbits) - 1) << rlo);
+  edk_ctx *ctx = op->ctx;
+  const uint32_t D = (uint32_t)b->D;
+  uint32_t *d_rk = nullptr;
+  if (hipMalloc((void **)&S->d_rtab, RK_NT * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&d_rk, (size_t)D * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipFree(d_rk);
+    return false;
+  }
+  uint32_t last = 0;
+  bool ok = hipMemcpyAsync(S->d_rtab, T.data(), RK_NT * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) ==
+            hipSuccess;
+  if (ok) {
+    k_rank_keys<<<(D + 255) / 256, 256, 0, ctx->stream>>>(S->d_keys32, D, S->d_rtab, A.dmask, rlo, A.rng, d_rk);
+    ok = hipMemcpyAsync(&last, d_rk + (D - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+         hipStreamSynchronize(ctx->stream) == hipSuccess;
+  }
+  const uint32_t ngran = last / RK_GRAN + 1;
+  ok = ok && hipMalloc((void **)&S->d_gran, (size_t)ngran * sizeof(uint4)) == hipSuccess;
+  if (ok) {
+    k_gran_build<<<(ngran + 255) / 256, 256, 0, ctx->stream>>>(d_rk, D, ngran, S->d_gran);
+    ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
+  }
+  (void)hipFree(d_rk);
+  if (!ok) return false;
+  A.gran = S->d_gran;
+  A.ngran = ngran;
+  A.rtab = S->d_rtab;
+  return true;
+}
+
 static bool fast_setup_build(edk_op *op) {
   FastState *S = fast_state(op);
   const edk_basis *b = op->basis;
@@ -578,6 +686,7 @@ static bool fast_setup_build(edk_op *op) {
   A.Rrow = b->d_R;
   A.region = nullptr;
   A.fine = nullptr;
+  A.gran = nullptr;
   {
     // two-level directory (4096 regions of 2^20 key values); fine slots
     // ~ one per representative; entries are 16-bit offsets from the region's
@@ -620,6 +729,7 @@ static bool fast_setup_build(edk_op *op) {
         }
       }
       if (fine.size() >= (1u << 27)) fits = false;
+      if (fits && !rank_setup(op, S, keys)) return false;
       if (fits) {
         if (hipMalloc((void **)&S->d_region, region.size() * sizeof(uint2)) != hipSuccess ||
             hipMalloc((void **)&S->d_fine, fine.size() * sizeof(uint16_t)) != hipSuccess)
@@ -709,6 +819,8 @@ static void launch_bal_lt(const FastArgs &A, int dtype, void *y, const void *x, 
 template <int KIND, int PH>
 static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st) {
   if (A.chunk) {
+    // the rank directory when it applies (real phases, run-length canonicalisers)
+    if (KIND != EDK_PROJECTED && PH != 2 && edk_fast_launch_rk(A, dtype, y, x, acc, st)) return;
     // rotation flavour: L = 32 (v_alignbit), 16 <= L < 32 with unit cell 1
     // (run-length canonicaliser on L-bit words), anything else (full search)
     if (A.L == 32 && A.A == 1) launch_bal_lt<KIND, PH, 32>(A, dtype, y, x, acc, st);
