@@ -1,9 +1,11 @@
 // edk_runtime.hip -- contexts, errors and device-memory helpers of the C ABI.
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <unordered_map>
 
+#include "edk_blockcache.h"
 #include "edk_internal.h"
 
 static thread_local std::string g_last_error;
@@ -58,25 +60,18 @@ extern "C" int edk_ctx_create(int device, void *stream, edk_ctx **out) {
   return EDK_OK;
 }
 
-// Cached allocations are rounded up to size classes (1/8 of the enclosing
-// power of two above 64 MiB, at most 12.5 % extra), so a block freed by one
-// job serves the next job of a similar sector.
-static size_t size_class(size_t b) {
-  if (b < ((size_t)64 << 20)) return (b + 255) & ~(size_t)255;
-  const int e = 63 - __builtin_clzll((unsigned long long)b);
-  const size_t g = (size_t)1 << (e - 3);
-  return (b + g - 1) & ~(g - 1);
-}
-
-// Per-device cache of free blocks by size class.  Blocks stay cached for the
-// life of the process; an allocation that finds the device full releases the
-// cache and retries.
-struct BlockCache {
-  std::mutex mu;
-  std::map<size_t, std::vector<void *>> free_blocks;
-  std::unordered_map<void *, size_t> owned;  // every block the cache handed out or holds
-  size_t cached = 0;
+// The device block cache (edk_blockcache.h) over hipMalloc / hipFree, one per
+// device.
+struct HipRaw {
+  static int alloc(void **p, size_t bytes) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) return RAW_OK;
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? RAW_OOM : RAW_ERR;
+  }
+  static void free(void *p) { (void)hipFree(p); }
 };
+typedef BlockCacheT<HipRaw> BlockCache;
 static std::mutex g_caches_mu;
 static std::map<int, BlockCache *> g_caches;
 
@@ -87,49 +82,17 @@ static BlockCache &cache_of(int device) {
   return *bc;
 }
 
-static void cache_release(BlockCache &bc) {
-  std::lock_guard<std::mutex> lk(bc.mu);
-  for (auto &kv : bc.free_blocks)
-    for (void *q : kv.second) {
-      (void)hipFree(q);
-      bc.owned.erase(q);
-    }
-  bc.free_blocks.clear();
-  bc.cached = 0;
-}
-
-size_t edk_cache_bytes(int device) {
-  BlockCache &bc = cache_of(device);
-  std::lock_guard<std::mutex> lk(bc.mu);
-  return bc.cached;
-}
+size_t edk_cache_bytes(int device) { return (size_t)cache_of(device).stats().cached_bytes; }
 
 hipError_t edk_dalloc(edk_ctx *c, void **p, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (!c->cache_alloc) return hipMalloc(p, bytes);
-  const size_t b = size_class(bytes);
-  BlockCache &bc = cache_of(c->device);
-  {
-    std::lock_guard<std::mutex> lk(bc.mu);
-    auto it = bc.free_blocks.find(b);
-    if (it != bc.free_blocks.end() && !it->second.empty()) {
-      *p = it->second.back();
-      it->second.pop_back();
-      bc.cached -= b;
-      return hipSuccess;
-    }
-  }
-  hipError_t e = hipMalloc(p, b);
-  if (e == hipErrorOutOfMemory) {
-    (void)hipGetLastError();
-    cache_release(bc);
-    e = hipMalloc(p, b);
-  }
-  if (e == hipSuccess) {
-    std::lock_guard<std::mutex> lk(bc.mu);
-    bc.owned[*p] = b;
-  }
-  return e;
+  size_t released = 0;
+  const int e = cache_of(c->device).alloc(p, bytes, &released);
+  if (released)
+    fprintf(stderr, "[edk] device %d: block cache released (%zu bytes) after an allocation of %zu bytes failed\n",
+            c->device, released, bytes);
+  return e == RAW_OK ? hipSuccess : (e == RAW_OOM ? hipErrorOutOfMemory : hipErrorUnknown);
 }
 
 void edk_dfree(edk_ctx *c, void *p) {
@@ -140,15 +103,19 @@ void edk_dfree(edk_ctx *c, void *p) {
   }
   // the block's last use was queued on this context's stream
   (void)hipStreamSynchronize(c->stream);
-  BlockCache &bc = cache_of(c->device);
-  std::lock_guard<std::mutex> lk(bc.mu);
-  auto it = bc.owned.find(p);
-  if (it == bc.owned.end()) {
-    (void)hipFree(p);
-    return;
-  }
-  bc.free_blocks[it->second].push_back(p);
-  bc.cached += it->second;
+  cache_of(c->device).release(p);
+}
+
+extern "C" int edk_cache_info(int device, edk_cache_stats *out) {
+  if (!out) return edk_fail(EDK_ERR_ARG, "edk_cache_info: null argument");
+  const BlockCacheStats st = cache_of(device).stats();
+  out->cached_bytes = st.cached_bytes;
+  out->owned_blocks = st.owned_blocks;
+  out->releases = st.releases;
+  out->released_bytes = st.released_bytes;
+  out->double_frees = st.double_frees;
+  out->foreign_frees = st.foreign_frees;
+  return EDK_OK;
 }
 
 extern "C" int edk_ctx_destroy(edk_ctx *c) {

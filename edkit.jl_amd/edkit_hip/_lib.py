@@ -24,7 +24,7 @@ VARIANT_NAMES = {0: "auto", 1: "generic", 2: "tile", 3: "lane2", 4: "lane4", 5: 
 SYMBOLS = [
     "edk_last_error", "edk_version",
     "edk_ctx_create", "edk_ctx_destroy", "edk_ctx_synchronize", "edk_ctx_stream",
-    "edk_malloc", "edk_free", "edk_memcpy_htod", "edk_memcpy_dtoh", "edk_memset",
+    "edk_malloc", "edk_free", "edk_memcpy_htod", "edk_memcpy_dtoh", "edk_memset", "edk_cache_info",
     "edk_basis_build", "edk_basis_upload", "edk_basis_dim", "edk_basis_order",
     "edk_basis_download", "edk_basis_index", "edk_basis_destroy",
     "edk_operator_create", "edk_operator_destroy", "edk_operator_count_offdiag",
@@ -45,6 +45,19 @@ SYMBOLS = [
     "edk_lindblad_default_params", "edk_lindblad_create", "edk_lindblad_evolve",
     "edk_lindblad_diagnostics", "edk_lindblad_records", "edk_lindblad_dim", "edk_lindblad_destroy",
 ]
+
+
+class CacheStats(C.Structure):
+    """edk_cache_stats: diagnostics of the device block cache."""
+    _fields_ = [(n, C.c_int64) for n in ("cached_bytes", "owned_blocks", "releases", "released_bytes",
+                                          "double_frees", "foreign_frees")]
+
+
+def cache_info(device=0):
+    """The device block cache's counters as a dict (edk_cache_info)."""
+    st = CacheStats()
+    check(lib().edk_cache_info(device, C.byref(st)))
+    return {n: getattr(st, n) for n, _ in CacheStats._fields_}
 
 
 class EdkError(RuntimeError):
@@ -151,6 +164,7 @@ def lib():
         "edk_memcpy_htod": (I32, [P, P, P, SZ]),
         "edk_memcpy_dtoh": (I32, [P, P, P, SZ]),
         "edk_memset": (I32, [P, P, I32, SZ]),
+        "edk_cache_info": (I32, [I32, C.POINTER(CacheStats)]),
         "edk_basis_build": (I32, [P, C.POINTER(BasisSpec), PP]),
         "edk_basis_upload": (I32, [P, C.POINTER(BasisSpec), P, P, I64, PP]),
         "edk_basis_dim": (I64, [P]),

@@ -78,6 +78,16 @@ int edk_memcpy_htod(edk_ctx *ctx, void *dst, const void *src, size_t bytes);
 int edk_memcpy_dtoh(edk_ctx *ctx, void *dst, const void *src, size_t bytes);
 int edk_memset(edk_ctx *ctx, void *dptr, int value, size_t bytes);
 
+/* Diagnostics of the device block cache that serving contexts allocate from
+ * (no counterpart in the reference): bytes cached for reuse, blocks owned,
+ * whole-cache releases after an allocation failure and the bytes they
+ * returned, and the releases the cache refused (a block given back twice) or
+ * passed through (a block it never handed out). */
+typedef struct edk_cache_stats {
+  int64_t cached_bytes, owned_blocks, releases, released_bytes, double_frees, foreign_frees;
+} edk_cache_stats;
+int edk_cache_info(int device, edk_cache_stats *out);
+
 /* ---- bases -------------------------------------------------------------- */
 /* Keyword arguments of the reference constructors.  N < 0 means `nothing`.
  *   EDK_TENSOR              TensorBasis(L, base)            AbstractBasis.jl:234-238
