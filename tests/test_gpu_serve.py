@@ -58,13 +58,13 @@ def test_results_same_alone_and_in_mix(gpu):
     for (kind, H, kw), (ra, va), (rm, vm) in zip(jobs, alone, mixed):
         assert ra["status"] == 0 and rm["status"] == 0, (ra["message"], rm["message"])
         assert np.array_equal(va, vm), kind
+        for f in ("E0", "resid", "steps", "matvecs", "overlap_re", "overlap_im", "norm"):
+            assert ra[f] == rm[f], (kind, f)
     # the block cache the servers allocated from: no block given back twice,
     # none it never handed out
     from edkit_hip._lib import cache_info
     ci = cache_info(0)
     assert ci["double_frees"] == 0 and ci["foreign_frees"] == 0, ci
-        for f in ("E0", "resid", "steps", "matvecs", "overlap_re", "overlap_im", "norm"):
-            assert ra[f] == rm[f], (kind, f)
     # the direct entry points agree (their single-vector kernel rounds the
     # products in another order than the server's block launch)
     for (kind, H, kw), (ra, va) in zip(jobs, alone):
