@@ -399,6 +399,8 @@ CanonDev edk_basis::canon() const {
   c.gdesc = d_gdesc;
   c.gdata = d_gdata;
   c.gshift = d_gshift;
+  c.ptab = d_ptab;
+  c.pbits = pbits;
   return c;
 }
 
@@ -477,6 +479,7 @@ static void free_basis_mem(edk_basis *b) {
   hipFree(b->d_gdesc);
   hipFree(b->d_gdata);
   hipFree(b->d_gshift);
+  hipFree(b->d_ptab);
 }
 
 template <typename T>
@@ -555,10 +558,70 @@ static int add_generator(edk_basis *b, int order, int k, const int32_t *perm, co
     }
     b->h_ph.push_back(c);
   }
+  if (b->ngen == 0) {
+    b->h_perm0.assign(perm, perm + L);
+    b->h_inv0.assign(L, 0);
+    if (inv) b->h_inv0.assign(inv, inv + L);
+  }
   int d[EDK_GDESC] = {type, order, param, inv_slot, layer_off, (int)lm.size(), ph_off, 0};
   for (int q = 0; q < EDK_GDESC; q++) b->h_gdesc.push_back(d[q]);
   b->ngen++;
   return EDK_OK;
+}
+
+// Table of the pruned Abelian walk (edk_canon_abelian_pruned): applies with
+// two or three generators, none a Benes network, generator 0 of order <= 32
+// mapping the sites 1..k (the top k fields) onto themselves, for the largest
+// such k with k w <= 10 (at most 1024 entries).  Generator 0 is applied here
+// as the device applies it: fields move to their permuted sites, then the
+// inversion mask.
+static void build_prune_table(edk_basis *b, std::vector<uint2> &tab) {
+  tab.clear();
+  b->pbits = 0;
+  if (b->kind != EDK_ABELIAN || b->ngen < 2 || b->ngen > 3 || (int)b->h_perm0.size() != b->L) return;
+  for (int g = 0; g < b->ngen; g++)
+    if (b->h_gdesc[EDK_GDESC * g] == GEN_BENES) return;
+  const int L = b->L, w = b->w, n0 = b->h_gdesc[1];
+  if (n0 < 1 || n0 > 32) return;
+  int k = 0;
+  for (int kk = 1; kk < L && kk * w <= 10; kk++) {
+    bool inv_ok = true;
+    for (int i = 1; i <= kk && inv_ok; i++) inv_ok = b->h_perm0[i - 1] >= 1 && b->h_perm0[i - 1] <= kk;
+    if (inv_ok) k = kk;
+  }
+  if (k == 0) return;
+  const int tot = L * w, bits = k * w, sh = tot - bits;
+  const uint64_t fm = (1ull << w) - 1;
+  uint64_t invmask = 0, lowf = 0;
+  for (int i = 1; i <= L; i++) {
+    lowf |= 1ull << ((L - i) * w);
+    if (b->h_inv0[i - 1]) invmask |= ((b->base == 3) ? 2ull : fm) << ((L - i) * w);
+  }
+  auto apply0 = [&](uint64_t s) {
+    uint64_t r = 0;
+    for (int i = 1; i <= L; i++) r |= ((s >> ((L - i) * w)) & fm) << ((L - b->h_perm0[i - 1]) * w);
+    if (invmask) {
+      if (b->base != 3) r ^= invmask;
+      else r ^= invmask & ~((r & lowf) << 1);
+    }
+    return r;
+  };
+  tab.assign((size_t)1 << bits, make_uint2(0, 0));
+  for (uint64_t v = 0; v < ((uint64_t)1 << bits); v++) {
+    uint64_t c = v << sh;
+    uint32_t mn = 0xffffffffu, mask = 0;
+    for (int a0 = 0; a0 < n0; a0++) {
+      const uint32_t top = (uint32_t)(c >> sh);
+      if (top < mn) {
+        mn = top;
+        mask = 0;
+      }
+      if (top == mn) mask |= 1u << a0;
+      c = apply0(c);
+    }
+    tab[v] = make_uint2(mn, mask);
+  }
+  b->pbits = bits;
 }
 
 // Validate the spec and fill the canonicaliser tables (host + device).
@@ -708,6 +771,10 @@ static int setup_basis(edk_ctx *ctx, const edk_basis_spec *spec, edk_basis *b) {
   if (int rc = upload_vec(ctx, b->h_gdesc, &b->d_gdesc)) return rc;
   if (int rc = upload_vec(ctx, b->h_gdata, &b->d_gdata)) return rc;
   if (int rc = upload_vec(ctx, b->h_gshift, &b->d_gshift)) return rc;
+  std::vector<uint2> ptab;
+  build_prune_table(b, ptab);
+  if (!ptab.empty())
+    if (int rc = upload_vec(ctx, ptab, &b->d_ptab)) return rc;
   return EDK_OK;
 }
 

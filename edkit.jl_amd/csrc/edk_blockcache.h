@@ -3,10 +3,11 @@
 // Allocations are rounded up to size classes (1/8 of the enclosing power of
 // two above 64 MiB, at most 12.5 % extra), so a block freed by one job serves
 // the next job of a similar sector.  Blocks stay cached for the life of the
-// process; an allocation that finds the device full releases every cached
-// block and retries.  The raw allocator is a template parameter so the same
-// logic runs over hipMalloc in the library and over a host heap in the stress
-// test (tests/cache_stress.cpp):
+// process; an allocation that finds the device full releases cached blocks,
+// the largest first, until the released bytes cover it, retries, and
+// releases the rest of the cache only if that still fails.  The raw allocator
+// is a template parameter so the same logic runs over hipMalloc in the
+// library and over a host heap in the stress test (tests/cache_stress.cpp):
 //   struct Raw { static int alloc(void **p, size_t bytes);   // RAW_OK / RAW_OOM / RAW_ERR
 //                static void free(void *p); };
 //
@@ -16,7 +17,7 @@
 //     double free and ignored, so the block is never handed out twice);
 //   - only cached blocks are released to the raw allocator; a handed-out block
 //     stays owned until its holder gives it back;
-//   - every release of the whole cache is counted (and reported on stderr by
+//   - every release of cached blocks is counted (and reported on stderr by
 //     the library).
 #pragma once
 #include <cstddef>
@@ -32,7 +33,7 @@ enum { RAW_OK = 0, RAW_OOM = 1, RAW_ERR = 2 };
 struct BlockCacheStats {
   int64_t cached_bytes = 0;    // bytes held in the cached lists
   int64_t owned_blocks = 0;    // blocks allocated through the cache and not yet released to the raw allocator
-  int64_t releases = 0;        // whole-cache releases after an allocation failure
+  int64_t releases = 0;        // cache releases after an allocation failure (largest blocks first, or all)
   int64_t released_bytes = 0;  // bytes they returned
   int64_t double_frees = 0;    // releases of a block that was already cached (ignored)
   int64_t foreign_frees = 0;   // releases of a block the cache never handed out (passed to the raw allocator)
@@ -73,9 +74,13 @@ struct BlockCacheT {
     }
     int e = Raw::alloc(p, b);
     if (e == RAW_OOM) {
-      const size_t r = release_all();
+      size_t r = release_largest(b);
+      if (r) e = Raw::alloc(p, b);
+      if (e == RAW_OOM) {
+        r += release_all();
+        e = Raw::alloc(p, b);
+      }
       if (released) *released = r;
-      e = Raw::alloc(p, b);
     }
     if (e == RAW_OK) {
       std::lock_guard<std::mutex> lk(mu);
@@ -103,6 +108,32 @@ struct BlockCacheT {
     free_blocks[it->second].push_back(p);
     free_set.insert(p);
     st.cached_bytes += (int64_t)it->second;
+  }
+
+  // Returns cached blocks to the raw allocator, largest size class first,
+  // until at least `need` bytes are released (or the cache is empty); counts
+  // as one release.  The bytes released.
+  size_t release_largest(size_t need) {
+    std::lock_guard<std::mutex> lk(mu);
+    size_t bytes = 0;
+    for (auto it = free_blocks.rbegin(); it != free_blocks.rend() && bytes < need; ++it) {
+      std::vector<void *> &v = it->second;
+      while (!v.empty() && bytes < need) {
+        void *q = v.back();
+        v.pop_back();
+        Raw::free(q);
+        owned.erase(q);
+        free_set.erase(q);
+        bytes += it->first;
+        st.cached_bytes -= (int64_t)it->first;
+      }
+    }
+    if (bytes) {
+      st.owned_blocks = (int64_t)owned.size();
+      st.releases++;
+      st.released_bytes += (int64_t)bytes;
+    }
+    return bytes;
   }
 
   // Returns every cached block to the raw allocator; handed-out blocks are

@@ -78,6 +78,11 @@ struct CanonDev {
   const int *gdesc;     // per generator: type, order, shift/param, ph offset, layer offset, nlayers
   const uint64_t *gdata;// Benes layer masks, inversion masks
   const int *gshift;    // Benes layer shifts
+  // pruned Abelian walk (null: the full walk): generator 0 maps the top
+  // pbits of the word onto themselves; per value v of those bits, the least
+  // top block over generator 0's powers and the mask of the powers reaching it
+  const uint2 *ptab;
+  int pbits;
 };
 
 struct SearchDev {
@@ -114,6 +119,10 @@ struct edk_basis {
   std::vector<int> h_gdesc;
   std::vector<uint64_t> h_gdata;
   std::vector<int> h_gshift;
+  uint2 *d_ptab = nullptr;   // pruned Abelian walk table (CanonDev::ptab)
+  int pbits = 0;
+  std::vector<int32_t> h_perm0;  // generator 0's permutation and inversion flags
+  std::vector<uint8_t> h_inv0;
   std::vector<uint64_t> pw;  // base^i
   int refcount = 0;
   bool uploaded = false;  // table came from the caller: gather R, never infer it
@@ -379,7 +388,123 @@ __device__ __forceinline__ CanonOut edk_canon_abelian3(const CanonDev &C, uint64
   return o;
 }
 
+// The same walk pruned by the top block (CanonDev::ptab; two or three
+// generators, none of them a Benes network).  Generator 0 maps the top pbits
+// onto themselves, so an image g0^a0 g'(s) has the top block
+// g0^a0(top(g'(s))) and the least top block over a0 is a table entry.  Pass 1
+// finds the least top block V over the outer elements g' = g1^a1 g2^a2; only
+// images with top block V can be the minimum or equal it, and pass 2 compares
+// just those (the table's mask of a0) in odometer order, keeping the first
+// strict minimum.  The elements mapping s onto its representative are a coset
+// of the stabiliser, so counting the images equal to the minimum gives the
+// stabiliser order the norm needs.
+// With at most EDK_PRUNE_OUT outer elements they are held in registers and
+// their table entries loaded together.
+#define EDK_PRUNE_OUT 8
+template <typename W>
+__device__ __forceinline__ CanonOut edk_canon_abelian_pruned(const CanonDev &C, uint64_t s64) {
+  const GenReg g0 = edk_load_gen(C, 0), g1 = edk_load_gen(C, 1);
+  const GenReg g2 = C.ngen > 2 ? edk_load_gen(C, 2) : g1;
+  const int n0 = g0.order, n1 = g1.order, n2 = C.ngen > 2 ? g2.order : 1;
+  const int sh = C.L * C.w - C.pbits;
+  const W s = (W)s64;
+  W best = s;
+  int be = 0, cnt = 0;
+  if (n1 * n2 <= EDK_PRUNE_OUT) {
+    // outer elements in odometer order (generator 1 fastest), then their
+    // table entries, all loads in flight together
+    W co[EDK_PRUNE_OUT];
+    W c1 = s, c2 = s;
+    int a1 = 0;
+#pragma unroll
+    for (int i = 0; i < EDK_PRUNE_OUT; i++) {
+      co[i] = c1;
+      if (++a1 < n1) {
+        c1 = edk_apply_genreg<W>(C, g1, c1);
+      } else {
+        a1 = 0;
+        c2 = edk_apply_genreg<W>(C, g2, c2);
+        c1 = c2;
+      }
+    }
+    uint2 te[EDK_PRUNE_OUT];
+#pragma unroll
+    for (int i = 0; i < EDK_PRUNE_OUT; i++) te[i] = C.ptab[i < n1 * n2 ? (uint32_t)(co[i] >> sh) : 0u];
+    uint32_t vmin = 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < EDK_PRUNE_OUT; i++) vmin = i < n1 * n2 ? min(vmin, te[i].x) : vmin;
+#pragma unroll
+    for (int i = 0; i < EDK_PRUNE_OUT; i++) {
+      uint32_t mask = (i < n1 * n2 && te[i].x == vmin) ? te[i].y : 0u;
+      W c0 = co[i];
+      int a0 = 0;
+      while (mask) {
+        const int nx = __ffs(mask) - 1;
+        mask &= mask - 1;
+        for (; a0 < nx; a0++) c0 = edk_apply_genreg<W>(C, g0, c0);
+        const int e = a0 + n0 * i;
+        if (cnt == 0 || c0 < best) {
+          best = c0;
+          be = e;
+          cnt = 1;
+        } else if (c0 == best) {
+          cnt++;
+        }
+      }
+    }
+  } else {
+    uint32_t vmin = 0xffffffffu;
+    W c2 = s;
+    for (int a2 = 0; a2 < n2; a2++) {
+      W c1 = c2;
+      for (int a1 = 0; a1 < n1; a1++) {
+        vmin = min(vmin, C.ptab[(uint32_t)(c1 >> sh)].x);
+        if (a1 + 1 < n1) c1 = edk_apply_genreg<W>(C, g1, c1);
+      }
+      if (a2 + 1 < n2) c2 = edk_apply_genreg<W>(C, g2, c2);
+    }
+    c2 = s;
+    for (int a2 = 0; a2 < n2; a2++) {
+      W c1 = c2;
+      for (int a1 = 0; a1 < n1; a1++) {
+        const uint2 t = C.ptab[(uint32_t)(c1 >> sh)];
+        if (t.x == vmin) {
+          uint32_t mask = t.y;
+          W c0 = c1;
+          int a0 = 0;
+          while (mask) {
+            const int nx = __ffs(mask) - 1;
+            mask &= mask - 1;
+            for (; a0 < nx; a0++) c0 = edk_apply_genreg<W>(C, g0, c0);
+            const int e = a0 + n0 * (a1 + n1 * a2);
+            if (cnt == 0 || c0 < best) {
+              best = c0;
+              be = e;
+              cnt = 1;
+            } else if (c0 == best) {
+              cnt++;
+            }
+          }
+        }
+        if (a1 + 1 < n1) c1 = edk_apply_genreg<W>(C, g1, c1);
+      }
+      if (a2 + 1 < n2) c2 = edk_apply_genreg<W>(C, g2, c2);
+    }
+  }
+  const int m0 = be % n0 + 1, m1 = (be / n0) % n1 + 1, m2 = be / (n0 * n1) + 1;
+  CanonOut o;
+  o.rep = (uint64_t)best;
+  o.ok = 1;
+  o.R = C.Rtab[cnt - 1];
+  double2 acc = C.ph[C.gdesc[6] + m0 - 1];
+  acc = cmul(acc, C.ph[C.gdesc[EDK_GDESC + 6] + m1 - 1]);
+  if (C.ngen > 2) acc = cmul(acc, C.ph[C.gdesc[2 * EDK_GDESC + 6] + m2 - 1]);
+  o.phase = acc;
+  return o;
+}
+
 __device__ __forceinline__ CanonOut edk_canon_abelian(const CanonDev &C, uint64_t s) {
+  if (C.ptab) return C.L * C.w <= 32 ? edk_canon_abelian_pruned<uint32_t>(C, s) : edk_canon_abelian_pruned<uint64_t>(C, s);
   if (C.ngen >= 1 && C.ngen <= 3 && C.gdesc[0] != GEN_BENES && (C.ngen < 2 || C.gdesc[EDK_GDESC] != GEN_BENES) &&
       (C.ngen < 3 || C.gdesc[2 * EDK_GDESC] != GEN_BENES))
     return C.L * C.w <= 32 ? edk_canon_abelian3<uint32_t>(C, s) : edk_canon_abelian3<uint64_t>(C, s);

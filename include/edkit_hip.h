@@ -80,7 +80,7 @@ int edk_memset(edk_ctx *ctx, void *dptr, int value, size_t bytes);
 
 /* Diagnostics of the device block cache that serving contexts allocate from
  * (no counterpart in the reference): bytes cached for reuse, blocks owned,
- * whole-cache releases after an allocation failure and the bytes they
+ * releases of cached blocks after an allocation failure and the bytes they
  * returned, and the releases the cache refused (a block given back twice) or
  * passed through (a block it never handed out). */
 typedef struct edk_cache_stats {

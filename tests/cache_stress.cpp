@@ -1,6 +1,6 @@
 // cache_stress.cpp -- the device block cache (edk_blockcache.h) over a host
 // heap with a byte budget, driven by threads that allocate, hold, stamp and
-// give back blocks while the budget keeps forcing whole-cache releases.
+// give back blocks while the budget keeps forcing cache releases.
 //
 // Checked on every step:
 //   - a block is never held by two threads at once;
@@ -128,7 +128,7 @@ int main(int argc, char **argv) {
   for (auto &t : th) t.join();
   BlockCacheStats st = g_cache.stats();
   if (st.double_frees || st.foreign_frees) fail("unexpected double or foreign release under threads");
-  if (st.releases == 0) fail("the budget never forced a whole-cache release (test too weak)");
+  if (st.releases == 0) fail("the budget never forced a cache release (test too weak)");
   const long releases_threads = (long)st.releases;
 
   // a second release of a cached block is refused and never hands it out twice
@@ -156,7 +156,7 @@ int main(int argc, char **argv) {
     if (!g_live.empty() || g_used != 0) fail("raw blocks alive after everything was released");
   }
   if (st.owned_blocks != 0 || st.cached_bytes != 0) fail("cache still owns blocks after release_all");
-  printf("threads %d iterations %ld: whole-cache releases %ld, raw allocation failures %ld, errors %d\n", nt, iters,
+  printf("threads %d iterations %ld: cache releases %ld, raw allocation failures %ld, errors %d\n", nt, iters,
          releases_threads, (long)g_raw_oom, g_errors.load());
   return g_errors.load() ? 1 : 0;
 }

@@ -1,7 +1,7 @@
 """The device block cache's host logic (edk_blockcache.h) under threads, built
 with the host compiler and its address/undefined-behaviour and thread
 sanitizers, over a budgeted host heap (tests/cache_stress.cpp): a block is
-never handed out twice or released while held, whole-cache releases after an
+never handed out twice or released while held, cache releases after an
 allocation failure free only cached blocks, a second release of a block is
 refused, and releasing everything leaves nothing alive."""
 import os
