@@ -12,7 +12,7 @@ roofline: algorithmic bytes per matvec
 exactly on the device) over the kernel's mean duration, timed with HIP events
 on the stream the kernel runs on.  roofline.traffic: HBM bytes per launch from
 the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same kernel and
-workload (profiles/matvec_bal_L32_tp_k0.md).  cpu_baseline: the oracle's threaded scatter
+workload (profiles/matvec_rk_L32_tp_k0.md).  cpu_baseline: the oracle's threaded scatter
 mul on a strided sample of columns of the same matvec, scaled to H*psi/s.
 """
 import argparse
@@ -33,12 +33,17 @@ HBM_PEAK_GBS = 8000.0
 L, NUP, DELTA = 32, 16, 1.0
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the default matvec kernel on this
 # workload, scaled by the copy-probe calibration (a measured figure, not live)
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "matvec_bal_L32_tp_k0.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "matvec_rk_L32_tp_k0.md")
 
 
-# the sources the default matvec kernel is compiled from
+# the sources the default matvec kernel (k_matvec_rk) and its setup are compiled from
 KERNEL_SOURCES = [os.path.join(ROOT, "edkit.jl_amd", "csrc", f)
-                  for f in ("edk_fast.hip", "edk_fast.h", "edk_internal.h")]
+                  for f in ("edk_fast_rk.hip", "edk_fast.hip", "edk_fast.h", "edk_internal.h")]
+
+
+# the sources of the generic gather kernel (k_matvec_gtile) and its canonicalisers
+GENERIC_SOURCES = [os.path.join(ROOT, "edkit.jl_amd", "csrc", f)
+                   for f in ("edk_matvec.hip", "edk_matvec.h", "edk_internal.h")]
 
 
 def kernel_source_hash(paths=KERNEL_SOURCES):
@@ -811,7 +816,8 @@ def c7_workload(args, ev):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(os.path.join(ROOT, "profiles", "matvec_generic_torus6x5_custom.md")),
+                     "traffic": profiled_traffic(os.path.join(ROOT, "profiles", "matvec_generic_torus6x5_custom.md"),
+                                                 GENERIC_SOURCES),
                      "bytes_alg_per_launch": b_alg,
                      "kernel_ms": kern_ms},
         "basis_build_s": t_build,
