@@ -208,11 +208,13 @@ def c4_workload(args, ev):
     """Batched uneven sectors: J1-J2 ring L=28 J2=0.3 N=14, all 28 momentum
     sectors, plus the spin-1 BLBQ chain L=16 beta=0.2 Sz=0 k=0; a block of 8
     seeded complex vectors per sector, ComplexF32 matvec.  One step = H_s X_s
-    for every sector s.  Timed four ways: one vector at a time (8 x 29
-    launches), one block launch per sector (29), one graph launch, and one
-    graph launch after autotuning each sector's schedule; then once more with
-    every sector's matrix assembled and cached (sparse! semantics: block
-    products through CSR SpMM).  value is the fastest graph mode."""
+    for every sector s.  Timed: one vector at a time (8 x 29 launches), one
+    block launch per sector (29), one graph with a launch per sector, one
+    graph with the sectors of one block kernel fused into launches of up to
+    12 sectors (the 26 complex-momentum sectors share three), the same after autotuning each
+    sector's schedule; then once more with every sector's matrix assembled and
+    cached (sparse! semantics: block products through CSR SpMM).  value is the
+    fastest graph mode."""
     import edkit_hip as edk
     from edkit_hip.matvec import device_operator, matvec_into
     sys.path.insert(0, ROOT)
@@ -266,8 +268,13 @@ def c4_workload(args, ev):
     batch = edk.MatvecBatch([(H, Xd, Yd) for H, Xd, Yd, _, _ in items], streams=4)
     res["graph_4_branches_ms"] = timed(batch.launch, steps)
     batch.close()
+    batch = edk.MatvecBatch([(H, Xd, Yd) for H, Xd, Yd, _, _ in items], fuse=False)
+    res["graph_launch_per_sector_ms"] = timed(batch.launch, steps)
+    res["graph_launch_per_sector_launches"] = batch.launches
+    batch.close()
     batch = edk.MatvecBatch([(H, Xd, Yd) for H, Xd, Yd, _, _ in items])
     res["graph_ms"] = timed(batch.launch, steps)
+    res["graph_launches"] = batch.launches
     batch.close()
     tuned = {}
     for H, _, _, _, _ in items:
@@ -290,7 +297,9 @@ def c4_workload(args, ev):
     for H, _, _, _, _ in items:
         device_operator(H).release_csr()
     from edkit_hip._lib import VARIANT_NAMES
-    modes = {"graph_ms": "one graph, matrix-free", "graph_autotuned_ms": "one graph, autotuned schedules",
+    modes = {"graph_ms": "one graph, matrix-free (sectors of one block kernel fused, up to 12 a launch)",
+             "graph_launch_per_sector_ms": "one graph, matrix-free, one launch per sector",
+             "graph_autotuned_ms": "one graph, autotuned schedules",
              "graph_assembled_ms": "one graph over the assembled matrices (sparse! cache)"}
     best_mode = min(modes, key=lambda k: res[k])
     best = res[best_mode]
@@ -308,6 +317,10 @@ def c4_workload(args, ev):
                      "bytes_alg_per_step": b_alg},
         "modes_ms": res,
         "value_mode": modes[best_mode],
+        "matrix_free": {"ms": min(res["graph_ms"], res["graph_launch_per_sector_ms"], res["graph_autotuned_ms"]),
+                        "frac": b_alg / (min(res["graph_ms"], res["graph_launch_per_sector_ms"],
+                                             res["graph_autotuned_ms"]) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "fused_vs_launch_per_sector": res["graph_launch_per_sector_ms"] / res["graph_ms"]},
         "speedup_graph_vs_one_at_a_time": res["one_vector_at_a_time_ms"] / best,
         "autotuned_schedules": {VARIANT_NAMES[k]: v for k, v in tuned.items()},
         "basis_build_s": t_build,

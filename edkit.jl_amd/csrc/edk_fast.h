@@ -900,6 +900,24 @@ struct FastState {
 
 // block-of-vectors launch (edk_fast_blk.hip); nv in {2, 4, 8}
 bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, int nv, int acc, hipStream_t st);
+// Sectors of one fused block launch, passed by value in the kernel
+// arguments (pointers read from there are known to be global memory; read
+// from a device array they would take flat loads): sector k's arguments, its
+// row-interleaved x and y and its first workgroup (bstart[n] = total
+// workgroups).  12 sectors keep the struct under 3.5 KB.
+#define MS_MAX 12
+struct MultiSec {
+  int n;
+  uint32_t bstart[MS_MAX + 1];
+  const void *x[MS_MAX];
+  void *y[MS_MAX];
+  FastArgs a[MS_MAX];
+};
+// phase mode of a sector's block launch (-1: none); sectors fuse when their
+// kind, mode, length class and unit cell agree
+int edk_fast_blk_mode(const FastArgs &A);
+// one launch over the M.n sectors of M (the template follows M.a[0])
+bool edk_fast_launch_blk_multi(const MultiSec &M, int dtype, int nv, int acc, hipStream_t st);
 // one-vector launch with the rank directory (edk_fast_rk.hip); false when it
 // does not apply (no directory, complex phases, not a run-length canonicaliser)
 bool edk_fast_launch_rk(const FastArgs &A, int dtype, void *y, const void *x, int acc, hipStream_t st);

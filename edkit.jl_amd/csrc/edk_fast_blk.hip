@@ -8,8 +8,8 @@
 // threads, one per vector, so each representative lookup serves all NV
 // columns and the NV gathered values of a target form one contiguous segment.
 template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
-__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT *__restrict__ x,
-                                                             VT *__restrict__ y, int accumulate) {
+__device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, const VT *__restrict__ x,
+                                             VT *__restrict__ y, int accumulate) {
   constexpr bool CPLX = (PH == 2);
   __shared__ double sR[2 * 64];
   __shared__ double2 sP[PH != 0 ? 64 : 1];
@@ -40,7 +40,6 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
   FastArgs B = A;
   B.Rtab = sR;
   if (PH != 0 && KIND == EDK_TRANSLATIONAL) B.ph = sP;
-  const uint32_t cb = bal_chunk(blockIdx.x, A.nchunk, A.xcdg);
   const uint32_t r0 = A.chunk[cb], nrow = A.chunk[cb + 1] - r0;
   // phase 1: row threads count and write their targets; diagonal and 1/R_i
   uint32_t s = 0, n = 0;
@@ -200,6 +199,88 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
       out.y = yi;
     }
     y[i] = out;
+  }
+}
+
+template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
+__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT *__restrict__ x,
+                                                             VT *__restrict__ y, int accumulate) {
+  bal_blk_body<KIND, PH, LT, VT, ST, NV>(A, bal_chunk(blockIdx.x, A.nchunk, A.xcdg), x, y, accumulate);
+}
+
+// Several sectors' block products in one launch (the fused form of a
+// multi-sector batch): sector k (arguments M.a[k]) owns the workgroups
+// [bstart[k], bstart[k+1]),
+// a range padded to a multiple of 8 so that each sector's chunks keep the
+// XCD-grouped order of its own launch; padding workgroups exit at once.
+// Workgroups of small and large sectors run side by side, and the chip
+// drains once instead of once per sector.
+template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
+__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk_multi(MultiSec M, int accumulate) {
+  // the sector: a count over all range starts (independent scalar reads of
+  // the kernel arguments, not a chain of dependent ones)
+  int k = 0;
+#pragma unroll
+  for (int q = 1; q < MS_MAX; q++) k += (q < M.n && blockIdx.x >= M.bstart[q]) ? 1 : 0;
+  const FastArgs &A = M.a[k];
+  const uint32_t lb = blockIdx.x - M.bstart[k];
+  if (lb >= A.nchunk) return;
+  bal_blk_body<KIND, PH, LT, VT, ST, NV>(A, bal_chunk(lb, A.nchunk, A.xcdg), (const VT *)M.x[k], (VT *)M.y[k],
+                                          accumulate);
+}
+
+template <int KIND, int PH, int LT, int NV>
+static void launch_blk_multi(const MultiSec &M, int dtype, int acc, hipStream_t st) {
+  const uint32_t nb = M.bstart[M.n];
+  if (dtype == EDK_C64)
+    k_matvec_bal_blk_multi<KIND, PH, LT, double2, double, NV><<<nb, BAL_TPB, 0, st>>>(M, acc);
+  else
+    k_matvec_bal_blk_multi<KIND, PH, LT, float2, float, NV><<<nb, BAL_TPB, 0, st>>>(M, acc);
+}
+
+template <int KIND, int PH, int NV>
+static void launch_blk_multi_lt(const MultiSec &M, int dtype, int acc, hipStream_t st) {
+  const FastArgs &A0 = M.a[0];
+  if (A0.L == 32 && A0.A == 1) launch_blk_multi<KIND, PH, 32, NV>(M, dtype, acc, st);
+  else if (A0.A == 1 && A0.L >= 16 && KIND != EDK_PROJECTED) launch_blk_multi<KIND, PH, 1, NV>(M, dtype, acc, st);
+  else launch_blk_multi<KIND, PH, 0, NV>(M, dtype, acc, st);
+}
+
+template <int KIND, int PH>
+static bool launch_blk_multi_nv(const MultiSec &M, int dtype, int nv, int acc, hipStream_t st) {
+  switch (nv) {
+    case 2: launch_blk_multi_lt<KIND, PH, 2>(M, dtype, acc, st); return true;
+    case 4: launch_blk_multi_lt<KIND, PH, 4>(M, dtype, acc, st); return true;
+    case 8: launch_blk_multi_lt<KIND, PH, 8>(M, dtype, acc, st); return true;
+    default: return false;
+  }
+}
+
+// the phase mode (0 real positive, 1 real signs, 2 complex) a sector's block
+// launch uses; -1 when it has no block launch
+int edk_fast_blk_mode(const FastArgs &A) {
+  if (!A.chunk) return -1;
+  switch (A.kind) {
+    case EDK_TRANSLATION_PARITY: return (A.kp == 1 && A.P == 1) ? 0 : 1;
+    case EDK_TRANSLATIONAL: return A.kp == 0 ? 0 : (2 * A.kp == A.len ? 1 : 2);
+    case EDK_PROJECTED: return 0;
+    default: return -1;
+  }
+}
+
+bool edk_fast_launch_blk_multi(const MultiSec &M, int dtype, int nv, int acc, hipStream_t st) {
+  if (M.n < 1 || M.n > MS_MAX) return false;
+  const FastArgs &A0 = M.a[0];
+  switch (A0.kind) {
+    case EDK_TRANSLATION_PARITY:
+      if (edk_fast_blk_mode(A0) == 0) return launch_blk_multi_nv<EDK_TRANSLATION_PARITY, 0>(M, dtype, nv, acc, st);
+      return launch_blk_multi_nv<EDK_TRANSLATION_PARITY, 1>(M, dtype, nv, acc, st);
+    case EDK_TRANSLATIONAL:
+      if (edk_fast_blk_mode(A0) == 0) return launch_blk_multi_nv<EDK_TRANSLATIONAL, 0>(M, dtype, nv, acc, st);
+      if (edk_fast_blk_mode(A0) == 1) return launch_blk_multi_nv<EDK_TRANSLATIONAL, 1>(M, dtype, nv, acc, st);
+      return launch_blk_multi_nv<EDK_TRANSLATIONAL, 2>(M, dtype, nv, acc, st);
+    case EDK_PROJECTED: return launch_blk_multi_nv<EDK_PROJECTED, 0>(M, dtype, nv, acc, st);
+    default: return false;
   }
 }
 

@@ -31,7 +31,7 @@ SYMBOLS = [
     "edk_matvec", "edk_matvec_host", "edk_matvec_multi",
     "edk_operator_assemble", "edk_operator_download_csr", "edk_operator_release_csr",
     "edk_operator_set_variant", "edk_operator_autotune",
-    "edk_batch_create", "edk_batch_launch", "edk_batch_destroy",
+    "edk_batch_create", "edk_batch_create_ex", "edk_batch_launches", "edk_batch_launch", "edk_batch_destroy",
     "edk_server_default_params", "edk_server_create", "edk_server_create_ex", "edk_server_submit",
     "edk_server_wait", "edk_server_clock", "edk_server_stats", "edk_server_batch_stats", "edk_server_destroy",
     "edk_vec_scale", "edk_vec_axpy", "edk_vec_dot", "edk_vec_nrm2", "edk_vec_dot_noalloc",
@@ -184,6 +184,8 @@ def lib():
         "edk_operator_set_variant": (I32, [P, I32, I64, I32]),
         "edk_operator_autotune": (I32, [P, I32, I64, I32, C.POINTER(C.c_int), C.POINTER(D)]),
         "edk_batch_create": (I32, [P, I32, C.POINTER(BatchItem), I64, I32, PP]),
+        "edk_batch_create_ex": (I32, [P, I32, C.POINTER(BatchItem), I64, I32, I32, PP]),
+        "edk_batch_launches": (I64, [P]),
         "edk_batch_launch": (I32, [P]),
         "edk_batch_destroy": (I32, [P]),
         "edk_server_create": (I32, [I32, I32, SZ, PP]),

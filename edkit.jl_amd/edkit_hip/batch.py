@@ -17,9 +17,11 @@ from .matvec import device_operator
 class MatvecBatch:
     """items: sequence of (H, x, y) or (H, x, y, accumulate) with device arrays
     x, y of shape (D,) or (D, nb), all of one dtype and one context.  streams:
-    graph branches (0: one for sectors of >= 2^20 row-vectors, else 8)."""
+    graph branches (0: one for sectors of >= 2^20 row-vectors, else 8).
+    fuse: block products of sectors that take the same specialised block
+    kernel run as one launch over all their chunks (else one launch each)."""
 
-    def __init__(self, items, streams=0):
+    def __init__(self, items, streams=0, fuse=True):
         if not items:
             raise ValueError("empty batch")
         self._keep = []
@@ -48,8 +50,14 @@ class MatvecBatch:
         self.ctx = ctx
         self.dtype = dtype
         h = C.c_void_p()
-        check(lib().edk_batch_create(ctx.h, dtype_code(dtype), arr, len(items), int(streams), C.byref(h)))
+        check(lib().edk_batch_create_ex(ctx.h, dtype_code(dtype), arr, len(items), int(streams),
+                                        1 if fuse else 0, C.byref(h)))
         self.h = h
+
+    @property
+    def launches(self):
+        """Kernel launches the batch's graph holds."""
+        return int(lib().edk_batch_launches(self.h))
 
     def launch(self):
         """One graph launch (asynchronous on the context stream)."""

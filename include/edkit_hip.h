@@ -231,6 +231,15 @@ typedef struct {
 typedef struct edk_batch edk_batch;
 int edk_batch_create(edk_ctx *ctx, int dtype, const edk_batch_item *items, int64_t n, int streams,
                      edk_batch **out);
+/* The same with flags: EDK_BATCH_FUSE (edk_batch_create's default) runs the
+ * block products of sectors that take the same specialised block kernel as
+ * one launch over all their chunks (up to 12 sectors a launch); without it
+ * every item keeps its own launch.  No counterpart in the reference (mul(H, M) per sector,
+ * Operator.jl:354-370, is the unit a batch is made of). */
+#define EDK_BATCH_FUSE 1
+int edk_batch_create_ex(edk_ctx *ctx, int dtype, const edk_batch_item *items, int64_t n, int streams, int flags,
+                        edk_batch **out);
+int64_t edk_batch_launches(const edk_batch *b); /* kernel launches the batch's graph holds */
 int edk_batch_launch(edk_batch *b); /* one graph launch on the context stream, asynchronous */
 int edk_batch_destroy(edk_batch *b);
 

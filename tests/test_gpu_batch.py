@@ -93,11 +93,13 @@ def test_autotune_picks_a_valid_schedule(gpu):
     assert v == _lib.EDK_VARIANT_GENERIC
 
 
-@pytest.mark.parametrize("streams", [1, 4])
-def test_multi_sector_graph_batch(gpu, streams):
+@pytest.mark.parametrize("streams,fuse", [(1, True), (4, True), (1, False)])
+def test_multi_sector_graph_batch(gpu, streams, fuse):
     """All 16 momentum sectors of a J1-J2 ring L=16 N=8 (blocks of 8, FP32)
     plus a spin-1 BLBQ sector, in one graph; every replay equals the eager
-    matvecs bitwise and the oracle to FP32 rounding."""
+    matvecs bitwise and the oracle to FP32 rounding.  Fused, the 14 complex-
+    phase sectors share two launches (at most 12 sectors a launch; k = 0,
+    k = 8 and the spin-1 sector keep their own)."""
     import edkit_hip as edk
     items, eager, checks = [], [], []
     for k in range(16):
@@ -116,7 +118,8 @@ def test_multi_sector_graph_batch(gpu, streams):
     items.append((H3, X3, Y3))
     eager.append((H3 @ X3).to_host())
     checks.append((len(items) - 1, orc.Op(orc.Basis("translational", 8, base=3, N=8, k=0), H3.flatten())))
-    batch = edk.MatvecBatch(items, streams=streams)
+    batch = edk.MatvecBatch(items, streams=streams, fuse=fuse)
+    assert batch.launches == (5 if fuse else 17)
     for rep in range(3):
         for _, _, Y in items:
             Y.zero_()
@@ -132,6 +135,37 @@ def test_multi_sector_graph_batch(gpu, streams):
         for c in range(8):
             yo = op @ Xh[:, c].astype(np.complex128)
             assert np.linalg.norm(Yh[:, c] - yo) <= 2e-6 * np.linalg.norm(yo)
+    batch.close()
+
+
+def test_fused_batch_sectors_of_both_parities(gpu):
+    """Fused block launches over translation-parity sectors of two lengths
+    (ComplexF64, blocks of 4, accumulating): the sectors of one phase mode
+    share a launch and every result equals its own sector's launch bitwise
+    and the oracle to 1e-12."""
+    import edkit_hip as edk
+    items, eager, refs = [], [], []
+    for L, k, p in ((16, 0, 1), (18, 0, 1), (16, 0, -1), (18, 9, 1), (18, 9, -1), (16, 8, 1)):
+        B = edk.TranslationParityBasis(L=L, N=L // 2, k=k, p=p)
+        H = xxz(B, 0.7) if L == 16 else j1j2(B, 0.4)
+        X0 = block(B.D, 4, 31 * L + k + p, np.complex128)
+        Y0 = block(B.D, 4, 77 * L + k + p, np.complex128)
+        X = edk.DeviceArray.from_host(X0)
+        Y = edk.DeviceArray.from_host(Y0)
+        items.append((H, X, Y, True))
+        eager.append(Y0 + (H @ X).to_host())
+        refs.append((orc.Op(orc.Basis("translation_parity", L, N=L // 2, k=k, p=p), H.flatten()), X0, Y0))
+    batch = edk.MatvecBatch(items, streams=1)
+    # phase mode 0 (k = 0, p = +1) and mode 1 (the rest): two launches
+    assert batch.launches == 2
+    batch.launch()
+    gpu.synchronize()
+    for (H, X, Y, _), ye, (op, X0, Y0) in zip(items, eager, refs):
+        Yh = Y.to_host()
+        assert np.array_equal(Yh, ye)
+        for c in range(4):
+            yo = Y0[:, c] + op @ X0[:, c]
+            assert np.linalg.norm(Yh[:, c] - yo) <= 1e-12 * np.linalg.norm(yo)
     batch.close()
 
 
