@@ -898,6 +898,11 @@ struct FastState {
 };
 
 
+// rank directory over sorted 32-bit representatives (edk_fast.hip); A.gran
+// stays null when it does not apply; false only on a device error
+bool edk_rank_build(edk_ctx *ctx, int L, const std::vector<uint32_t> &keys, const uint32_t *d_keys32, FastArgs &A,
+                    uint4 **d_gran, uint32_t **d_rtab);
+
 // block-of-vectors launch (edk_fast_blk.hip); nv in {2, 4, 8}
 bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, int nv, int acc, hipStream_t st);
 // Sectors of one fused block launch, passed by value in the kernel

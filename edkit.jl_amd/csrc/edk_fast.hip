@@ -533,19 +533,16 @@ static bool fast_setup(edk_op *op) {
   return ok;
 }
 
-// Rank directory for the run-length canonicalisers (unit cell 1, 16 <= L <=
-// 32, translation kinds): applies when every representative has one
-// popcount; the domain leaves out bit 0 when every key has it set and the top
-// bit when every key has it clear (true of smallest rotations).  Returns
-// false only on a device error; the directory is left out when it does not
-// apply.
-static bool rank_setup(edk_op *op, FastState *S, const std::vector<uint32_t> &keys) {
-  const edk_basis *b = op->basis;
-  FastArgs &A = S->args;
+// Rank directory over sorted 32-bit representatives of L sites: applies
+// when every representative has one popcount; the domain leaves out bit 0
+// when every key has it set and the top bit when every key has it clear (true
+// of smallest rotations).  Fills A's rank fields (A.gran stays null when the
+// directory does not apply) and the two device arrays.  Returns false only on
+// a device error.
+bool edk_rank_build(edk_ctx *ctx, int L, const std::vector<uint32_t> &keys, const uint32_t *d_keys32, FastArgs &A,
+                    uint4 **d_gran, uint32_t **d_rtab) {
   A.gran = nullptr;
-  if (b->A != 1 || b->L < 16 || b->L > 32 || (b->kind != EDK_TRANSLATION_PARITY && b->kind != EDK_TRANSLATIONAL))
-    return true;
-  const int L = b->L;
+  if (keys.empty() || L < 1 || L > 32) return true;
   const uint32_t top = 1u << (L - 1);
   const int pc = __builtin_popcount(keys[0]);
   bool all1 = true, all0 = true;
@@ -583,34 +580,43 @@ static bool rank_setup(edk_op *op, FastState *S, const std::vector<uint32_t> &ke
   A.fix1 = all1 ? 1u : 0u;
   A.fix0 = all0 ? top : 0u;
   A.dmask = (uint32_t)((((uint64_t)1 << rbits) - 1) << rlo);
-  edk_ctx *ctx = op->ctx;
-  const uint32_t D = (uint32_t)b->D;
+  const uint32_t D = (uint32_t)keys.size();
   uint32_t *d_rk = nullptr;
-  if (hipMalloc((void **)&S->d_rtab, RK_NT * sizeof(uint32_t)) != hipSuccess ||
+  if (hipMalloc((void **)d_rtab, RK_NT * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void **)&d_rk, (size_t)D * sizeof(uint32_t)) != hipSuccess) {
     (void)hipFree(d_rk);
     return false;
   }
   uint32_t last = 0;
-  bool ok = hipMemcpyAsync(S->d_rtab, T.data(), RK_NT * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) ==
+  bool ok = hipMemcpyAsync(*d_rtab, T.data(), RK_NT * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) ==
             hipSuccess;
   if (ok) {
-    k_rank_keys<<<(D + 255) / 256, 256, 0, ctx->stream>>>(S->d_keys32, D, S->d_rtab, A.dmask, rlo, A.rng, d_rk);
+    k_rank_keys<<<(D + 255) / 256, 256, 0, ctx->stream>>>(d_keys32, D, *d_rtab, A.dmask, rlo, A.rng, d_rk);
     ok = hipMemcpyAsync(&last, d_rk + (D - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
          hipStreamSynchronize(ctx->stream) == hipSuccess;
   }
   const uint32_t ngran = last / RK_GRAN + 1;
-  ok = ok && hipMalloc((void **)&S->d_gran, (size_t)ngran * sizeof(uint4)) == hipSuccess;
+  ok = ok && hipMalloc((void **)d_gran, (size_t)ngran * sizeof(uint4)) == hipSuccess;
   if (ok) {
-    k_gran_build<<<(ngran + 255) / 256, 256, 0, ctx->stream>>>(d_rk, D, ngran, S->d_gran);
+    k_gran_build<<<(ngran + 255) / 256, 256, 0, ctx->stream>>>(d_rk, D, ngran, *d_gran);
     ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
   }
   (void)hipFree(d_rk);
   if (!ok) return false;
-  A.gran = S->d_gran;
+  A.gran = *d_gran;
   A.ngran = ngran;
-  A.rtab = S->d_rtab;
+  A.rtab = *d_rtab;
   return true;
+}
+
+// The directory for the run-length canonicalisers: unit cell 1, 16 <= L <=
+// 32, translation kinds.
+static bool rank_setup(edk_op *op, FastState *S, const std::vector<uint32_t> &keys) {
+  const edk_basis *b = op->basis;
+  S->args.gran = nullptr;
+  if (b->A != 1 || b->L < 16 || b->L > 32 || (b->kind != EDK_TRANSLATION_PARITY && b->kind != EDK_TRANSLATIONAL))
+    return true;
+  return edk_rank_build(op->ctx, b->L, keys, S->d_keys32, S->args, &S->d_gran, &S->d_rtab);
 }
 
 static bool fast_setup_build(edk_op *op) {
@@ -834,7 +840,10 @@ static void launch_fast(const FastArgs &A, int dtype, void *y, const void *x, in
 
 bool edk_matvec_prepare(edk_op *op) {
   const bool f = !op->disable_fast && fast_setup(op);
-  if (!f) (void)edk_generic_setup(op);
+  if (!f) {
+    if (!op->disable_fast) (void)edk_perm_prepare(op);
+    (void)edk_generic_setup(op);
+  }
   return f;
 }
 
@@ -889,7 +898,9 @@ int edk_fast_set_variant(edk_op *op, int dtype, int64_t nb, int variant) {
   if (variant >= EDK_VARIANT_LANE2 && nb != 1)
     return edk_fail(EDK_ERR_ARG, "lane schedules apply to one vector only");
   FastState *S = fast_state(op);
-  if (variant != EDK_VARIANT_AUTO && variant != EDK_VARIANT_GENERIC && !edk_matvec_prepare(op))
+  // the Abelian two-site kernel has the tile form for one vector only
+  const bool perm = variant == EDK_VARIANT_TILE && nb == 1 && !op->disable_fast && edk_perm_prepare(op);
+  if (variant != EDK_VARIANT_AUTO && variant != EDK_VARIANT_GENERIC && !edk_matvec_prepare(op) && !perm)
     return edk_fail(EDK_ERR_UNSUPPORTED, "operator/basis not eligible for the specialised kernels");
   S->tuned[dtype == EDK_C64 ? 0 : 1][nb] = variant;
   return EDK_OK;
@@ -912,6 +923,8 @@ int edk_fast_autotune(edk_op *op, int dtype, int64_t nb, int reps, int *variant,
       cand.push_back(EDK_VARIANT_LANE4);
       cand.push_back(EDK_VARIANT_LANE8);
     }
+  } else if (nb == 1 && !op->disable_fast && edk_perm_prepare(op)) {
+    cand.push_back(EDK_VARIANT_TILE);
   }
   void *x = nullptr, *y = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -46,6 +46,8 @@ struct edk_op {
   uint32_t ngchunk = 0;
   std::atomic<int> gstate{0};     // generic tile setup: 0 not yet, 1 built, -1 not applicable
   std::atomic<int> fstate{0};     // specialised kernel setup: 0 not yet, 1 built, -1 not applicable
+  void *perm = nullptr;           // state of the Abelian two-site kernel (edk_perm.hip)
+  std::atomic<int> pstate{0};     // its setup: 0 not yet, 1 built, -1 not applicable
   size_t scratch_bytes = 0;
   std::vector<int> h_tdesc, h_mrow_off, h_rowptr, h_col;
   std::vector<double2> h_val;
@@ -64,6 +66,11 @@ __device__ __forceinline__ uint64_t edk_content_to_packed_d(uint64_t c, int L, i
 
 // Specialised kernels; returns true when it handled the call.
 void edk_fast_release(edk_op *op);
+// Abelian bases with two-site families (edk_perm.hip): one vector; false
+// when it does not apply or a schedule other than the tile form is chosen
+bool edk_matvec_perm(edk_op *op, int dtype, void *y, const void *x, int accumulate, hipStream_t st);
+bool edk_perm_prepare(edk_op *op);  // one-time setup (allocates; run before capture)
+void edk_perm_release(edk_op *op);
 // assembled sector matrix: SpMM for nb > 1 (false: not assembled / not applicable)
 void edk_csr_release(edk_op *op);
 bool edk_matvec_csr(edk_op *op, int dtype, void *y, const void *x, int64_t nb, int accumulate, hipStream_t st);

@@ -812,6 +812,7 @@ extern "C" int edk_operator_destroy(edk_op *op) {
   hipFree(op->d_scratch);
   hipFree(op->d_gchunk);
   edk_fast_release(op);
+  edk_perm_release(op);
   edk_csr_release(op);
   if (op->basis) op->basis->refcount--;
   delete op;
@@ -848,6 +849,10 @@ int edk_matvec_launch(edk_op *op, int dtype, void *y, const void *x, int64_t nb,
   // several right-hand sides with the assembled matrix cached (sparse!)
   if (nb > 1 && edk_matvec_csr(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
   if (edk_matvec_fast(op, dtype, y, x, nb, accumulate, st)) return EDK_OK;
+  if (nb == 1 && edk_matvec_perm(op, dtype, y, x, accumulate, st)) {
+    EDK_HIP(hipGetLastError());
+    return EDK_OK;
+  }
   if (launch_gtile(op, dtype, y, x, nb, accumulate, st)) {
     EDK_HIP(hipGetLastError());
     return EDK_OK;
