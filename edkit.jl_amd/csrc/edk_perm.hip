@@ -6,8 +6,10 @@
 //   - the basis is Abelian, base 2, L <= 32, built by the library, and every
 //     representative has one popcount (the rank directory, edk_fast.h);
 //   - one to three generators, none a Benes network, each of order <= 32 and
-//     the group of order <= 64; the pruned walk's table, when the basis has
-//     one, of at most 256 entries;
+//     the group of order <= 64; the walk is pruned when the basis has the
+//     table of least top blocks with at most 128 entries (top blocks of at
+//     most 7 bits) and generator 0 has order <= 16 with every power in at
+//     most four rotation groups;
 //   - the operator's terms are two-site terms with real matrices that group
 //     into at most four families: terms of one matrix whose first sites are
 //     distinct.  A family is a map p -> pi(p) of bit positions on its first
@@ -43,7 +45,8 @@
 #define PM_MAXGRP 4
 #define PM_MAXORD 32   // order of one generator
 #define PM_MAXGRPORD 64  // order of the group
-#define PM_MAXPT 256   // entries of the pruned walk's table held in LDS
+#define PM_MAXPT 128   // entries of the pruned walk's table held in LDS
+#define PM_MAXPOW 16   // order of generator 0 in the pruned walk
 #define PMM_ABSENT 0x4000u
 
 struct PermFam {
@@ -70,12 +73,22 @@ struct PermGen {
   int order;
 };
 
+// A power of generator 0 in the same form, read by each lane from LDS with
+// its own exponent: groups past the power's own are empty masks.
+struct __align__(16) PowGen {
+  uint32_t m[EDK_MAXSHIFTS];
+  uint32_t rl[EDK_MAXSHIFTS];
+  uint32_t inv, pad[3];
+};
+
 struct PermTab {
   int nfam;
   PermFam fam[PM_MAXFAM];
   int ngen;
   PermGen gen[3];
   int pbits;                          // pruned walk's table: 2^pbits entries (0: the full walk)
+  uint32_t rcp0;                      // ceil(2^16 / order of generator 0): e / n0 = (e rcp0) >> 16 for e < 64
+  PowGen pow0[PM_MAXPOW];             // generator 0's powers (pruned walk)
   double R[PM_MAXGRPORD];             // norm by stabiliser order
   double2 ph[3 * PM_MAXORD];          // phase of generator g's m-th power
 };
@@ -87,6 +100,7 @@ struct PermState {
   uint32_t *d_keys32 = nullptr, *d_chunk = nullptr, *d_rtab = nullptr;
   uint4 *d_gran = nullptr;
   bool real = true;  // every phase real
+  int npw = 1;       // the most rotation groups of any power of generator 0
 };
 
 __device__ __forceinline__ uint32_t rotl32(uint32_t v, uint32_t k) { return (v << k) | (v >> ((32u - k) & 31u)); }
@@ -126,17 +140,29 @@ __device__ __forceinline__ uint32_t perm_apply(int L, const Gen32 &r, uint32_t s
   return o ^ r.inv;
 }
 
+// NPW: at least the most groups any power has (a compile-time bound: a
+// uniform branch per group costs more than the group)
+template <int NPW>
+__device__ __forceinline__ uint32_t perm_apply_pow(const PowGen &g, uint32_t s) {
+  uint32_t o = rotl32(s & g.m[0], g.rl[0]);
+#pragma unroll
+  for (int q = 1; q < NPW; q++) o |= rotl32(s & g.m[q], g.rl[q]);
+  return o ^ g.inv;
+}
+
 // The Abelian walk over 32-bit words with the generators in registers:
 // the representative, the element number e = a0 + n0 (a1 + n1 a2) of the
 // first strict minimum (odometer order, generator 0 fastest) and the
 // stabiliser order.  PRUNE: only images whose top block is the least over
-// generator 0's powers are compared (pt: the table in LDS; see
-// edk_canon_abelian_pruned); the images equal to the minimum are a coset of
-// the stabiliser, so their number is its order.
-template <bool PRUNE>
-__device__ __forceinline__ uint32_t perm_canon(int L, const Gen32 &g0, const Gen32 &g1,
-                                               const Gen32 &g2, int n0, int n1, int n2, const uint2 *pt, int sh,
-                                               uint32_t s, int &be, int &cnt) {
+// generator 0's powers are compared (pt: the table in LDS, entries packed as
+// least block | mask of the powers reaching it << 8; the pruning of
+// edk_canon_abelian_pruned); each candidate is one power of generator 0 (pw)
+// applied to its outer image, and the images equal to the minimum are a
+// coset of the stabiliser, so their number is its order.
+template <bool PRUNE, int NPW>
+__device__ __forceinline__ void perm_canon(int L, const Gen32 &g0, const Gen32 &g1, const Gen32 &g2, int n0,
+                                           int n1, int n2, const uint32_t *pt, int sh, const PowGen *pw,
+                                           uint32_t rcp0, uint32_t s, uint32_t &rep, int &be, int &cnt) {
   uint32_t best = s;
   be = 0;
   cnt = 0;
@@ -176,29 +202,33 @@ __device__ __forceinline__ uint32_t perm_canon(int L, const Gen32 &g0, const Gen
         }
       }
     }
-    uint2 te[EDK_PRUNE_OUT];
+    uint32_t te[EDK_PRUNE_OUT];
 #pragma unroll
     for (int i = 0; i < EDK_PRUNE_OUT; i++) te[i] = pt[i < n1 * n2 ? (co[i] >> sh) : 0u];
     uint32_t vmin = 0xffffffffu;
 #pragma unroll
-    for (int i = 0; i < EDK_PRUNE_OUT; i++) vmin = i < n1 * n2 ? min(vmin, te[i].x) : vmin;
+    for (int i = 0; i < EDK_PRUNE_OUT; i++) vmin = i < n1 * n2 ? min(vmin, te[i] & 0xffu) : vmin;
+    // the candidates as one mask over element numbers e = a0 + n0 i (the
+    // group has at most 64 elements), taken in increasing e: the odometer
+    // order of the walk; each costs one power of generator 0
+    uint64_t cm = 0;
 #pragma unroll
-    for (int i = 0; i < EDK_PRUNE_OUT; i++) {
-      uint32_t mask = (i < n1 * n2 && te[i].x == vmin) ? te[i].y : 0u;
-      uint32_t c0 = co[i];
-      int a0 = 0;
-      while (mask) {
-        const int nx = __ffs(mask) - 1;
-        mask &= mask - 1;
-        for (; a0 < nx; a0++) c0 = perm_apply(L, g0, c0);
-        const int e = a0 + n0 * i;
-        if (cnt == 0 || c0 < best) {
-          best = c0;
-          be = e;
-          cnt = 1;
-        } else if (c0 == best) {
-          cnt++;
-        }
+    for (int i = 0; i < EDK_PRUNE_OUT; i++)
+      if (i < n1 * n2 && (te[i] & 0xffu) == vmin) cm |= (uint64_t)(te[i] >> 8) << (i * n0);
+    while (cm) {
+      const int e = __ffsll((unsigned long long)cm) - 1;
+      cm &= cm - 1;
+      const int i = (int)(((uint32_t)e * rcp0) >> 16);
+      uint32_t c = co[0];
+#pragma unroll
+      for (int k = 1; k < EDK_PRUNE_OUT; k++) c = i == k ? co[k] : c;
+      c = perm_apply_pow<NPW>(pw[e - i * n0], c);
+      if (cnt == 0 || c < best) {
+        best = c;
+        be = e;
+        cnt = 1;
+      } else if (c == best) {
+        cnt++;
       }
     }
   } else {
@@ -207,7 +237,7 @@ __device__ __forceinline__ uint32_t perm_canon(int L, const Gen32 &g0, const Gen
     for (int a2 = 0; a2 < n2; a2++) {
       uint32_t c1 = c2;
       for (int a1 = 0; a1 < n1; a1++) {
-        vmin = min(vmin, pt[c1 >> sh].x);
+        vmin = min(vmin, pt[c1 >> sh] & 0xffu);
         if (a1 + 1 < n1) c1 = perm_apply(L, g1, c1);
       }
       if (a2 + 1 < n2) c2 = perm_apply(L, g2, c2);
@@ -216,9 +246,9 @@ __device__ __forceinline__ uint32_t perm_canon(int L, const Gen32 &g0, const Gen
     for (int a2 = 0; a2 < n2; a2++) {
       uint32_t c1 = c2;
       for (int a1 = 0; a1 < n1; a1++) {
-        const uint2 t = pt[c1 >> sh];
-        if (t.x == vmin) {
-          uint32_t mask = t.y;
+        const uint32_t t = pt[c1 >> sh];
+        if ((t & 0xffu) == vmin) {
+          uint32_t mask = t >> 8;
           uint32_t c0 = c1;
           int a0 = 0;
           while (mask) {
@@ -240,11 +270,12 @@ __device__ __forceinline__ uint32_t perm_canon(int L, const Gen32 &g0, const Gen
       if (a2 + 1 < n2) c2 = perm_apply(L, g2, c2);
     }
   }
-  return best;
+  rep = best;
 }
 
 // PH 0: every phase real (coefficients real); PH 2: complex coefficients.
-template <int PH, bool PRUNE, typename VT, typename ST>
+// NPW: rotation groups of generator 0's powers (pruned walk; 2 or 4).
+template <int PH, bool PRUNE, int NPW, typename VT, typename ST>
 __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
     k_matvec_perm(FastArgs A, const PermTab *__restrict__ tab, const uint2 *__restrict__ ptab,
                   const VT *__restrict__ x, VT *__restrict__ y, int accumulate) {
@@ -259,7 +290,8 @@ __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
   __shared__ double sV[PM_MAXFAM * FAST_MAXENT];
   __shared__ double sR[PM_MAXGRPORD];
   __shared__ PT sPh[3 * PM_MAXORD];
-  __shared__ uint2 sPt[PRUNE ? PM_MAXPT : 1];
+  __shared__ uint32_t sPt[PRUNE ? PM_MAXPT : 1];  // least top block | mask of powers reaching it << 8
+  __shared__ PowGen sPw[PRUNE ? PM_MAXPOW : 1];
   __shared__ uint8_t sPart[PM_MAXFAM * 32];
   __shared__ uint32_t wsum[BAL_TPB / 64];
   uint32_t *const tw = reinterpret_cast<uint32_t *>(ub);
@@ -287,7 +319,11 @@ __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
     }
     if constexpr (PRUNE) {
       const int np = 1 << T.pbits;
-      for (int q = tid; q < np; q += BAL_TPB) sPt[q] = ptab[q];
+      for (int q = tid; q < np; q += BAL_TPB) {
+        const uint2 e = ptab[q];
+        sPt[q] = e.x | (e.y << 8);
+      }
+      if (tid < PM_MAXPOW) sPw[tid] = T.pow0[tid];
     }
   }
   const int psh = PRUNE ? A.L - T.pbits : 0;
@@ -336,26 +372,30 @@ __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
   __syncthreads();
   // phase 2a: representative, coefficient and rank of each of the thread's
   // targets (its own slots only)
+  auto finish = [&](uint32_t j, uint32_t rep, int be, int cnt) {
+    const uint32_t m = tm[j];
+    const int m0 = be % n0, m1 = (be / n0) % n1, m2 = be / (n0 * n1);
+    const double R = sR[cnt - 1], v = sV[m & 63u];
+    if constexpr (PH == 0) {
+      tc[j] = (ST)(v * sPh[m0] * sPh[PM_MAXORD + m1] * sPh[2 * PM_MAXORD + m2] * R);
+    } else {
+      const double2 p = cmul(cmul(sPh[m0], sPh[PM_MAXORD + m1]), sPh[2 * PM_MAXORD + m2]);
+      VT c;  // v conj(phase) R
+      c.x = (ST)(v * p.x * R);
+      c.y = (ST)(-v * p.y * R);
+      tc[j] = c;
+    }
+    if (rank_domain(A, rep)) tw[j] = rank_full(sT, (rep & A.dmask) >> A.rlo, A.rng);
+    else tm[j] = m | PMM_ABSENT;
+  };
 #pragma unroll 1
   for (int q = 0; q < BAL_K; q++) {
     const uint32_t j = tid + BAL_TPB * q;
     if (j < total) {
-      const uint32_t t = tw[j], m = tm[j];
+      uint32_t rep;
       int be, cnt;
-      const uint32_t rep = perm_canon<PRUNE>(A.L, g0, g1, g2, n0, n1, n2, sPt, psh, t, be, cnt);
-      const int m0 = be % n0, m1 = (be / n0) % n1, m2 = be / (n0 * n1);
-      const double R = sR[cnt - 1], v = sV[m & 63u];
-      if constexpr (PH == 0) {
-        tc[j] = (ST)(v * sPh[m0] * sPh[PM_MAXORD + m1] * sPh[2 * PM_MAXORD + m2] * R);
-      } else {
-        const double2 p = cmul(cmul(sPh[m0], sPh[PM_MAXORD + m1]), sPh[2 * PM_MAXORD + m2]);
-        VT c;  // v conj(phase) R
-        c.x = (ST)(v * p.x * R);
-        c.y = (ST)(-v * p.y * R);
-        tc[j] = c;
-      }
-      if (rank_domain(A, rep)) tw[j] = rank_full(sT, (rep & A.dmask) >> A.rlo, A.rng);
-      else tm[j] = m | PMM_ABSENT;
+      perm_canon<PRUNE, NPW>(A.L, g0, g1, g2, n0, n1, n2, sPt, psh, sPw, T.rcp0, tw[j], rep, be, cnt);
+      finish(j, rep, be, cnt);
     }
   }
   // phase 2b (each thread reads back only its own slots)
@@ -530,6 +570,54 @@ static uint32_t perm_partner_host(const PermFam &F, uint32_t s) {
   return r;
 }
 
+static uint32_t perm_apply_host(int L, const PermGen &G, uint32_t s) {
+  if (G.rev) s = __builtin_bitreverse32(s) >> (32 - L);
+  uint32_t o = 0;
+  for (int q = 0; q < G.n; q++) o |= rotl32_host(s & G.m[q], G.rl[q]);
+  return o ^ G.inv;
+}
+
+// Generator 0's powers as PowGen (bit permutation by rotation groups, then
+// the accumulated inversion g^a(0)); false when a power needs more than
+// EDK_MAXSHIFTS groups (then the walk is not pruned).
+static bool perm_powers(int L, PermTab &tab, int &npw) {
+  const PermGen &G = tab.gen[0];
+  if (G.order > PM_MAXPOW) return false;
+  uint32_t img[32];  // g^a of bit p, without the inversion
+  for (int p = 0; p < L; p++) img[p] = 1u << p;
+  uint32_t inv = 0;
+  for (int a = 0; a < G.order; a++) {
+    PowGen &P = tab.pow0[a];
+    memset(&P, 0, sizeof(P));
+    P.inv = inv;
+    int ng = 0;
+    for (int p = 0; p < L; p++) {
+      if (__builtin_popcount(img[p]) != 1) return false;
+      const uint32_t r = (uint32_t)(__builtin_ctz(img[p]) - p) & 31u;
+      int g = -1;
+      for (int k = 0; k < ng && g < 0; k++)
+        if (P.rl[k] == r) g = k;
+      if (g < 0) {
+        if (ng == EDK_MAXSHIFTS) return false;
+        g = ng++;
+        P.rl[g] = r;
+      }
+      P.m[g] |= 1u << p;
+    }
+    // next power: apply g to every bit image (the inversion is tracked apart)
+    for (int p = 0; p < L; p++) img[p] = perm_apply_host(L, G, img[p]) ^ G.inv;
+    inv = perm_apply_host(L, G, inv);
+  }
+  npw = 1;
+  for (int a = 0; a < G.order; a++)
+    for (int q = 0; q < EDK_MAXSHIFTS; q++)
+      if (tab.pow0[a].m[q]) npw = std::max(npw, q + 1);
+  tab.rcp0 = (65536u + (uint32_t)G.order - 1) / (uint32_t)G.order;
+  for (uint32_t e = 0; e < PM_MAXGRPORD; e++)
+    if (((e * tab.rcp0) >> 16) != e / (uint32_t)G.order) return false;
+  return true;
+}
+
 // The basis' generators as PermGen; false for a form this kernel lacks.
 static bool perm_gens(const edk_basis *b, PermTab &tab) {
   const int L = b->L;
@@ -595,7 +683,8 @@ static bool perm_build(edk_op *op, PermState *P) {
   memset(&tab, 0, sizeof(tab));
   if (!perm_families(op, tab) || !perm_gens(b, tab)) return false;
   // the pruned walk when the basis has its table and it fits in LDS
-  P->ptab = (b->d_ptab && b->pbits >= 1 && (1 << b->pbits) <= PM_MAXPT) ? b->d_ptab : nullptr;
+  P->ptab = (b->d_ptab && b->pbits >= 1 && (1 << b->pbits) <= PM_MAXPT && perm_powers(b->L, tab, P->npw)) ? b->d_ptab
+                                                                                                : nullptr;
   tab.pbits = P->ptab ? b->pbits : 0;
   for (int i = 0; i < b->order && i < PM_MAXGRPORD; i++) tab.R[i] = b->h_Rtab[i];
   P->real = true;
@@ -705,13 +794,13 @@ void edk_perm_release(edk_op *op) {
   op->pstate.store(0, std::memory_order_release);
 }
 
-template <int PH, bool PRUNE>
+template <int PH, bool PRUNE, int NPW>
 static void launch_perm(const PermState &P, int dtype, void *y, const void *x, int acc, hipStream_t st) {
   if (dtype == EDK_C64)
-    k_matvec_perm<PH, PRUNE, double2, double>
+    k_matvec_perm<PH, PRUNE, NPW, double2, double>
         <<<P.A.nchunk, BAL_TPB, 0, st>>>(P.A, P.d_tab, P.ptab, (const double2 *)x, (double2 *)y, acc);
   else
-    k_matvec_perm<PH, PRUNE, float2, float>
+    k_matvec_perm<PH, PRUNE, NPW, float2, float>
         <<<P.A.nchunk, BAL_TPB, 0, st>>>(P.A, P.d_tab, P.ptab, (const float2 *)x, (float2 *)y, acc);
 }
 
@@ -724,13 +813,15 @@ bool edk_matvec_perm(edk_op *op, int dtype, void *y, const void *x, int accumula
   }
   if (!edk_perm_prepare(op)) return false;
   const PermState &P = *(const PermState *)op->perm;
-  const bool prune = P.ptab != nullptr;
-  if (P.real) {
-    if (prune) launch_perm<0, true>(P, dtype, y, x, accumulate, st);
-    else launch_perm<0, false>(P, dtype, y, x, accumulate, st);
+  if (!P.ptab) {
+    if (P.real) launch_perm<0, false, 2>(P, dtype, y, x, accumulate, st);
+    else launch_perm<2, false, 2>(P, dtype, y, x, accumulate, st);
+  } else if (P.npw <= 2) {
+    if (P.real) launch_perm<0, true, 2>(P, dtype, y, x, accumulate, st);
+    else launch_perm<2, true, 2>(P, dtype, y, x, accumulate, st);
   } else {
-    if (prune) launch_perm<2, true>(P, dtype, y, x, accumulate, st);
-    else launch_perm<2, false>(P, dtype, y, x, accumulate, st);
+    if (P.real) launch_perm<0, true, EDK_MAXSHIFTS>(P, dtype, y, x, accumulate, st);
+    else launch_perm<2, true, EDK_MAXSHIFTS>(P, dtype, y, x, accumulate, st);
   }
   return true;
 }

@@ -79,8 +79,8 @@ TORI = [
     (4, 3, 6, 1, 2, 1, False),
     (5, 4, 10, 2, 3, 0, False),
     (6, 4, 12, 2, 1, None, False),
-    (6, 5, 9, 0, 0, None, False),
-    (6, 5, 9, 1, 2, None, False),
+    (6, 5, 7, 0, 0, None, False),
+    (6, 5, 7, 1, 2, None, False),
     (4, 4, 8, 1, 0, None, True),
     (6, 4, 11, 0, 2, None, True),
     (10, 3, 6, 0, 1, None, False),
