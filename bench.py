@@ -41,9 +41,9 @@ KERNEL_SOURCES = [os.path.join(ROOT, "edkit.jl_amd", "csrc", f)
                   for f in ("edk_fast_rk.hip", "edk_fast.hip", "edk_fast.h", "edk_internal.h")]
 
 
-# the sources of the generic gather kernel (k_matvec_gtile) and its canonicalisers
-GENERIC_SOURCES = [os.path.join(ROOT, "edkit.jl_amd", "csrc", f)
-                   for f in ("edk_matvec.hip", "edk_matvec.h", "edk_internal.h")]
+# the sources of the Abelian two-site kernel (k_matvec_perm) and the definitions it uses
+PERM_SOURCES = [os.path.join(ROOT, "edkit.jl_amd", "csrc", f)
+                for f in ("edk_perm.hip", "edk_fast.h", "edk_internal.h")]
 
 
 def kernel_source_hash(paths=KERNEL_SOURCES):
@@ -793,7 +793,8 @@ def c7_workload(args, ev):
     (L = 30), momentum (0, 0) with T_x and T_y given as site permutations
     (basis(...; symmetries); T_x is applied in the shift-and-mask form, T_y as
     a rotation of the packed word), N = 15.
-    One step = one ComplexF64 H*psi through the generic gather kernel;
+    One step = one ComplexF64 H*psi through the Abelian two-site kernel
+    (bit-parallel bond families, pruned group walk, rank-directory lookups);
     roofline bytes as for the default line (D (8 + 8 + 32) + NNZ_off (16 + 8)).
     The CPU leg times the oracle's threaded scatter on a column sample."""
     import edkit_hip as edk
@@ -829,8 +830,8 @@ def c7_workload(args, ev):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(os.path.join(ROOT, "profiles", "matvec_generic_torus6x5_custom.md"),
-                                                 GENERIC_SOURCES),
+                     "traffic": profiled_traffic(os.path.join(ROOT, "profiles", "matvec_perm_torus6x5_custom.md"),
+                                                 PERM_SOURCES),
                      "bytes_alg_per_launch": b_alg,
                      "kernel_ms": kern_ms},
         "basis_build_s": t_build,
