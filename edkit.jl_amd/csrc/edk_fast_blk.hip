@@ -215,6 +215,9 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk(FastArgs A, const VT
 // XCD-grouped order of its own launch; padding workgroups exit at once.
 // Workgroups of small and large sectors run side by side, and the chip
 // drains once instead of once per sector.
+// kernel arguments are limited to 4 KB (MultiSec plus the accumulate flag)
+static_assert(sizeof(MultiSec) + 16 <= 4096, "fused launch arguments exceed the kernel argument limit");
+
 template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
 __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_blk_multi(MultiSec M, int accumulate) {
   // the sector: a count over all range starts (independent scalar reads of
