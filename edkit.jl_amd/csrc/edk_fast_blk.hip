@@ -173,18 +173,31 @@ __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, con
     const uint32_t row = idx / NV, c = idx % NV;
     const uint32_t q0 = roff[row], q1 = q0 + rcnt[row];
     ST orr = 0, oi = 0;
-    for (uint32_t q = q0; q < q1; q++) {
-      const uint32_t jj = tgt[q];
-      if (jj == HASH_EMPTY) continue;
-      const VT xj = x[(size_t)jj * NV + c];
+    // four gathers in flight per thread; an absent target reads row 0 and
+    // adds nothing (terms are added in generator order either way)
+    auto term = [&](uint32_t q, const VT &xj) {
+      const bool hit = tgt[q] != HASH_EMPTY;
       if (CPLX) {
-        orr += tcr[q] * (ST)xj.x - tci[q] * (ST)xj.y;
-        oi += tcr[q] * (ST)xj.y + tci[q] * (ST)xj.x;
+        orr += hit ? tcr[q] * (ST)xj.x - tci[q] * (ST)xj.y : (ST)0;
+        oi += hit ? tcr[q] * (ST)xj.y + tci[q] * (ST)xj.x : (ST)0;
       } else {
-        orr += tcr[q] * (ST)xj.x;
-        oi += tcr[q] * (ST)xj.y;
+        orr += hit ? tcr[q] * (ST)xj.x : (ST)0;
+        oi += hit ? tcr[q] * (ST)xj.y : (ST)0;
       }
+    };
+    auto gather = [&](uint32_t q) {
+      const uint32_t jj = tgt[q];
+      return x[(size_t)(jj != HASH_EMPTY ? jj : 0u) * NV + c];
+    };
+    uint32_t q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      const VT x0 = gather(q), x1 = gather(q + 1), x2 = gather(q + 2), x3 = gather(q + 3);
+      term(q, x0);
+      term(q + 1, x1);
+      term(q + 2, x2);
+      term(q + 3, x3);
     }
+    for (; q < q1; q++) term(q, gather(q));
     const size_t i = (size_t)(r0 + row) * NV + c;
     const VT xi = x[i];
     const ST yr = rdg[row] * (ST)xi.x + orr * rinv[row];
@@ -570,12 +583,17 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
       dg += (ST)(sD[c][f * 4 + 0] * rpop[row][f * 4 + 0] + sD[c][f * 4 + 1] * rpop[row][f * 4 + 1] +
                  sD[c][f * 4 + 2] * rpop[row][f * 4 + 2] + sD[c][f * 4 + 3] * rpop[row][f * 4 + 3]);
     ST orr = 0, oi = 0;
-    for (uint32_t q = q0; q < q1; q++) {
+    // four gathers in flight per lane; an absent target reads row 0 and adds
+    // nothing.  Each product is formed as in the one-column path before it is
+    // added, in generator order, so a column's result does not depend on NV.
+    auto gather = [&](uint32_t q) {
       const uint32_t jq = tgt[q];
-      if (jq == HASH_EMPTY) continue;
-      const double v = sV[c][ent[q]];
+      return X[(size_t)(jq != HASH_EMPTY ? jq : 0u) * NV + c];
+    };
+    auto term = [&](uint32_t q, const VT &xj) {
+      const bool hit = tgt[q] != HASH_EMPTY;
+      const double v = sV[c][hit ? ent[q] : 0];
       const ST cr = (ST)(v * tcr[q]);
-      const VT xj = X[(size_t)jq * NV + c];
       ST pr, pi;
       if (CPLX) {
         const ST ci = (ST)(v * tci[q]);
@@ -585,9 +603,18 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
         pr = cr * (ST)xj.x;
         pi = cr * (ST)xj.y;
       }
-      orr += pr;
-      oi += pi;
+      orr += hit ? pr : (ST)0;
+      oi += hit ? pi : (ST)0;
+    };
+    uint32_t q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      const VT x0 = gather(q), x1 = gather(q + 1), x2 = gather(q + 2), x3 = gather(q + 3);
+      term(q, x0);
+      term(q + 1, x1);
+      term(q + 2, x2);
+      term(q + 3, x3);
     }
+    for (; q < q1; q++) term(q, gather(q));
     const VT xi = X[(size_t)(r0 + row) * NV + c];
     VT out;
     out.x = dg * (ST)xi.x + orr * rinv[row];

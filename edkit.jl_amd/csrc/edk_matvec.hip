@@ -457,15 +457,27 @@ __global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uin
       const int64_t i = (int64_t)r0 + rw;
       const uint32_t o = s_off[rw], cnt = s_n[rw];
       ST orr = 0, oi = 0;
-      for (uint32_t q = o; q < o + cnt; q++) {
+      // four gathers in flight per lane; an absent target reads row 0 and
+      // adds nothing (terms in order either way)
+      auto gather = [&](uint32_t q) {
         const uint64_t pq = word[q];
-        if (pq != GT_NONE) {
-          const VT cf = contr[q];
-          const VT xv = x[(int64_t)pq * nb + col + c];
-          orr += cf.x * (ST)xv.x - cf.y * (ST)xv.y;
-          oi += cf.x * (ST)xv.y + cf.y * (ST)xv.x;
-        }
+        return x[(int64_t)(pq != GT_NONE ? pq : 0) * nb + col + c];
+      };
+      auto term = [&](uint32_t q, const VT &xv) {
+        const bool hit = word[q] != GT_NONE;
+        const VT cf = contr[q];
+        orr += hit ? cf.x * (ST)xv.x - cf.y * (ST)xv.y : (ST)0;
+        oi += hit ? cf.x * (ST)xv.y + cf.y * (ST)xv.x : (ST)0;
+      };
+      uint32_t q = o;
+      for (; q + 4 <= o + cnt; q += 4) {
+        const VT x0 = gather(q), x1 = gather(q + 1), x2 = gather(q + 2), x3 = gather(q + 3);
+        term(q, x0);
+        term(q + 1, x1);
+        term(q + 2, x2);
+        term(q + 3, x3);
       }
+      for (; q < o + cnt; q++) term(q, gather(q));
       const VT dg = s_dg[rw];
       const VT xi = x[i * nb + col + c];
       const ST invR = A.kind != EDK_PROJECTED ? (ST)(1.0 / A.R[i]) : (ST)1;
