@@ -190,6 +190,13 @@ __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, con
       return x[(size_t)(jj != HASH_EMPTY ? jj : 0u) * NV + c];
     };
     uint32_t q = q0;
+    for (; q + 8 <= q1; q += 8) {
+      VT xs[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) xs[u] = gather(q + u);
+#pragma unroll
+      for (int u = 0; u < 8; u++) term(q + u, xs[u]);
+    }
     for (; q + 4 <= q1; q += 4) {
       const VT x0 = gather(q), x1 = gather(q + 1), x2 = gather(q + 2), x3 = gather(q + 3);
       term(q, x0);
