@@ -17,6 +17,10 @@ struct KrylovWork {
   int nblk = 1;
   double2 *d_part = nullptr;  // block partials
   double2 *d_s = nullptr;     // reduced coefficients
+  // device-driven Lanczos steps: alpha_j in [0, M), (beta_j, second sweep
+  // wanted) in [M, 2M), 1/beta of the last step in [2M], (first step the
+  // device loop could not settle, reason) in [2M + 1]; M = EDK_KRYLOV_MAXM
+  double2 *d_lz = nullptr;
   double2 *h_s = nullptr;     // pinned host mirror
   // slot: which of the context's two pinned scalar buffers to use
   int init(edk_ctx *c, int64_t n, int dtype, int maxm, int slot = 0);

@@ -157,6 +157,55 @@ __global__ void k_scale(VT *y, const VS *x, double s, int64_t n) {
   st(y, i, make_double2(b.x * s, b.y * s));
 }
 
+// y = x * Re(sp[0])  (a scale left on the device)
+template <typename VT>
+__global__ void k_scale_dev(VT *y, const VT *x, const double2 *sp, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = sp[0].x;
+  double2 b = ld(x, i);
+  st(y, i, make_double2(b.x * s, b.y * s));
+}
+
+// Device-driven Lanczos step j (KrylovWork::d_lz layout): the norm partials
+// of w after the first Gram-Schmidt sweep are reduced as k_reduce_part does,
+// then thread 0 takes the host loop's decisions on the same doubles: the
+// second-sweep test ||w'||^2 < (||w'||^2 + ||h||^2) / 2 over the sweep's
+// coefficients h = s[0..j), beta_j = sqrt(||w'||^2), the breakdown rule and
+// 1/beta_j.  The first step that wants a second sweep (reason 1) or breaks
+// down (reason 2) is recorded; the host takes over from there.
+#define LZ_M EDK_KRYLOV_MAXM
+__global__ void __launch_bounds__(RED_TPB) k_lz_step(const double2 *part, int nb, const double2 *s, int j,
+                                                     double2 *lz) {
+  double2 acc = make_double2(0, 0);
+  for (int b = threadIdx.x; b < nb; b += RED_TPB) {
+    acc.x += part[b].x;
+    acc.y += part[b].y;
+  }
+  const double2 r = block_sum2(acc);
+  if (threadIdx.x == 0) {
+    const double w2 = r.x;
+    double hh = 0.0;
+    for (int c = 0; c < j; c++) hh += s[c].x * s[c].x + s[c].y * s[c].y;
+    const bool second = w2 < 0.5 * (w2 + hh);
+    const double bj = sqrt(w2), aj = lz[j - 1].x;
+    lz[LZ_M + j - 1] = make_double2(bj, second ? 1.0 : 0.0);
+    lz[2 * LZ_M] = make_double2(1.0 / bj, 0.0);
+    const double sq = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+    if (lz[2 * LZ_M + 1].x == 0.0) {
+      if (second) lz[2 * LZ_M + 1] = make_double2((double)j, 1.0);
+      else if (bj <= sq * fmax(1.0, fabs(aj))) lz[2 * LZ_M + 1] = make_double2((double)j, 2.0);
+    }
+  }
+}
+
+// start of a device-driven extension from step m0: no step settled yet, and
+// beta_{m0} (from the host's array) where step m0 + 1 reads it
+__global__ void k_lz_start(double2 *lz, int m0, double beta_m0) {
+  lz[2 * LZ_M + 1] = make_double2(0.0, 0.0);
+  if (m0 >= 1) lz[LZ_M + m0 - 1] = make_double2(beta_m0, 0.0);
+}
+
 // part[c * nblk + b] = block b's share of conj(V[:, c]) . w,  c < j.
 // Columns go in groups of GEMV_CG: each thread loads w_i once per group and
 // keeps one accumulator per column, so w is read j / GEMV_CG times instead of
@@ -389,6 +438,8 @@ int KrylovWork::init(edk_ctx *c, int64_t n_, int dtype_, int maxm, int slot) {
   static_assert(ALPHA_SLOT < EDK_PINNED_SLOT, "coefficient slot must fit in the pinned buffer");
   if (edk_dalloc(ctx, (void **)&d_s, (NORM_SLOT + 1) * sizeof(double2)) != hipSuccess)
     return edk_fail(EDK_ERR_OOM, "krylov coefficients");
+  if (edk_dalloc(ctx, (void **)&d_lz, (2 * EDK_KRYLOV_MAXM + 2) * sizeof(double2)) != hipSuccess)
+    return edk_fail(EDK_ERR_OOM, "krylov step scalars");
   h_s = (double2 *)ctx->pinned + (size_t)slot * EDK_PINNED_SLOT;
   return EDK_OK;
 }
@@ -397,9 +448,11 @@ void KrylovWork::release() {
   if (ctx) {
     edk_dfree(ctx, d_part);
     edk_dfree(ctx, d_s);
+    edk_dfree(ctx, d_lz);
   }
   d_part = nullptr;
   d_s = nullptr;
+  d_lz = nullptr;
   h_s = nullptr;
 }
 
@@ -763,8 +816,9 @@ void PartialOrth::reset(int j) {
 // Returns 1 on lucky breakdown (beta set to 0), 0 otherwise; <0 on error.
 // po == nullptr: full reorthogonalisation at every step (the reference's
 // two-sweep Gram-Schmidt); otherwise partial reorthogonalisation.
-int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta,
-                   int *m, int m_target, int64_t *matvecs, int ncols, PartialOrth *po, double brk) {
+static int lanczos_extend_host(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha,
+                               double *beta, int *m, int m_target, int64_t *matvecs, int ncols, PartialOrth *po,
+                               double brk) {
   size_t es = K.esize();
   for (int j = *m + 1; j <= m_target; j++) {
     char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
@@ -822,6 +876,79 @@ int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, dou
     }
   }
   return 0;
+}
+
+// The same steps with full reorthogonalisation and the reference's breakdown
+// rule, their scalars left on the device (k_lz_step): the host queues every
+// step's kernels without waiting and synchronises once at the end.  Each
+// step runs the host loop's kernels on the same data in the same order, so
+// alpha, beta and the basis are the host loop's bit for bit.  A step that
+// wants a second Gram-Schmidt sweep is not settled here: the device steps
+// from it on are discarded and the host loop redoes them (rare; a Lanczos
+// vector is nearly orthogonal to the basis already).
+static int lanczos_extend_dev(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha,
+                              double *beta, int *m, int m_target, int64_t *matvecs, int ncols) {
+  const size_t es = K.esize();
+  const int m0 = *m;
+  hipStream_t s = K.ctx->stream;
+  double2 *lz = K.d_lz;
+  const int64_t n = K.n;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  const int dtype = K.dtype;
+  k_lz_start<<<1, 1, 0, s>>>(lz, m0, m0 >= 1 ? beta[m0 - 1] : 0.0);
+  for (int j = m0 + 1; j <= m_target; j++) {
+    char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
+    if (int rc = edk_matvec_launch(op, dtype, w, vj, 1, 0, s)) return -rc;
+    // w -= beta_{j-1} v_{j-1} and alpha_j = v_j^H w, into lz[j - 1]
+    if (j > 1) {
+      const char *vjm1 = (const char *)V + (size_t)(j - 2) * ldv * es;
+      DISPATCH(VT, k_axpy_dot_part<VT><<<K.nblk, RED_TPB, 0, s>>>((VT *)w, (const VT *)vjm1, make_double2(0, 0),
+                                                                  lz + LZ_M + (j - 2), -1.0, (const VT *)vj, n,
+                                                                  K.d_part));
+    } else {
+      DISPATCH(VT, k_dot_part<VT><<<K.nblk, RED_TPB, 0, s>>>((const VT *)vj, (const VT *)w, n, K.d_part));
+    }
+    k_reduce_part<<<1, RED_TPB, 0, s>>>(K.d_part, K.nblk, lz + (j - 1));
+    DISPATCH(VT, k_axpy_re_dev<VT><<<g, 256, 0, s>>>((VT *)w, (const VT *)vj, lz + (j - 1), -1.0, n));
+    // one sweep, then ||w||^2 and the step's scalars
+    if (int rc = K.cgs_sweep(V, ldv, j, w)) return -rc;
+    DISPATCH(VT, k_dot_part<VT><<<K.nblk, RED_TPB, 0, s>>>((const VT *)w, (const VT *)nullptr, n, K.d_part));
+    k_lz_step<<<1, RED_TPB, 0, s>>>(K.d_part, K.nblk, K.d_s, j, lz);
+    if (j + 1 <= ncols) {
+      char *vn = (char *)V + (size_t)j * ldv * es;
+      DISPATCH(VT, k_scale_dev<VT><<<g, 256, 0, s>>>((VT *)vn, (const VT *)w, lz + 2 * LZ_M, n));
+    }
+    if (hipGetLastError() != hipSuccess) return -edk_fail(EDK_ERR_HIP, "Lanczos step launch");
+  }
+  std::vector<double2> h(2 * LZ_M + 2);
+  if (hipMemcpyAsync(h.data(), lz, h.size() * sizeof(double2), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -edk_fail(EDK_ERR_HIP, "Lanczos step scalars");
+  const int stop = (int)h[2 * LZ_M + 1].x, why = (int)h[2 * LZ_M + 1].y;
+  const int last = stop ? (why == 1 ? stop - 1 : stop) : m_target;
+  for (int j = m0 + 1; j <= last; j++) {
+    alpha[j - 1] = h[j - 1].x;
+    beta[j - 1] = h[LZ_M + j - 1].x;
+  }
+  *matvecs += last - m0;
+  *m = last;
+  if (stop && why == 2) {
+    beta[stop - 1] = 0.0;
+    return 1;
+  }
+  if (stop) return lanczos_extend_host(op, K, V, ldv, w, alpha, beta, m, m_target, matvecs, ncols, nullptr, -1.0);
+  return 0;
+}
+
+// EDK_KRYLOV_HOST_STEPS=1 keeps every step on the host loop (the device-driven
+// steps are tested against it bit for bit)
+int lanczos_extend(edk_op *op, KrylovWork &K, void *V, int64_t ldv, void *w, double *alpha, double *beta, int *m,
+                   int m_target, int64_t *matvecs, int ncols, PartialOrth *po, double brk) {
+  const char *env = getenv("EDK_KRYLOV_HOST_STEPS");
+  const bool host_only = env && env[0] == '1';
+  if (!host_only && !po && brk < 0 && m_target - *m >= 2 && K.d_lz)
+    return lanczos_extend_dev(op, K, V, ldv, w, alpha, beta, m, m_target, matvecs, ncols);
+  return lanczos_extend_host(op, K, V, ldv, w, alpha, beta, m, m_target, matvecs, ncols, po, brk);
 }
 
 // ---------------------------------------------------------------------------
