@@ -151,7 +151,23 @@ __global__ void __launch_bounds__(256) k_lv_spmm_left(double2 *__restrict__ Y, c
   const int64_t c = blockIdx.y;
   const double2 *xc = X + c * d;
   double ar = 0.0, ai = 0.0;
-  for (int e = rp[r]; e < rp[r + 1]; e++) {
+  // four terms' loads in flight, added in order
+  const int e1 = rp[r + 1];
+  int e = rp[r];
+  for (; e + 4 <= e1; e += 4) {
+    double2 a[4], x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      a[u] = v[e + u];
+      x[u] = xc[ci[e + u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      ar += a[u].x * x[u].x - a[u].y * x[u].y;
+      ai += a[u].x * x[u].y + a[u].y * x[u].x;
+    }
+  }
+  for (; e < e1; e++) {
     const double2 a = v[e], x = xc[ci[e]];
     ar += a.x * x.x - a.y * x.y;
     ai += a.x * x.y + a.y * x.x;
@@ -173,7 +189,24 @@ __global__ void __launch_bounds__(256) k_lv_spmm_right(double2 *__restrict__ Y, 
   const int64_t c = blockIdx.y;
   const double sg = conj ? -1.0 : 1.0;
   double ar = 0.0, ai = 0.0;
-  for (int e = rp[c]; e < rp[c + 1]; e++) {
+  // four terms' loads in flight, added in order
+  const int e1 = rp[c + 1];
+  int e = rp[c];
+  for (; e + 4 <= e1; e += 4) {
+    double2 m[4], x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      m[u] = v[e + u];
+      x[u] = X[r + (int64_t)ci[e + u] * d];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double mi = sg * m[u].y;
+      ar += x[u].x * m[u].x - x[u].y * mi;
+      ai += x[u].x * mi + x[u].y * m[u].x;
+    }
+  }
+  for (; e < e1; e++) {
     const double2 m = v[e], x = X[r + (int64_t)ci[e] * d];
     const double mi = sg * m.y;
     ar += x.x * m.x - x.y * mi;
