@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(FAST_TPB, NB >= 8 ? 4 : (NB >= 4 ? 7 : 8))
 }
 
 template <int KIND, int PH, int LT, typename VT, typename ST>
-__global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && KIND != EDK_PROJECTED) ? 7 : 1) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
+__global__ void __launch_bounds__(BAL_TPB, ((LT == 32 || LT == 1) && PH == 0 && KIND != EDK_PROJECTED) ? 7 : (((LT == 32 || LT == 1) && PH == 2) ? 6 : 1)) k_matvec_bal(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate) {
   constexpr bool CPLX = (PH == 2);
   __shared__ double sR[2 * 64];

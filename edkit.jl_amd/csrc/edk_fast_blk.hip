@@ -363,7 +363,7 @@ bool edk_fast_launch_blk(const FastArgs &A, int dtype, void *y, const void *x, i
 // the same bits in a block of one as in any mix.
 // ---------------------------------------------------------------------------
 template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
-__global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M, const VT *__restrict__ X) {
+__global__ void __launch_bounds__(BAL_TPB, NV == 1 ? 6 : 1) k_matvec_bal_mc(FastArgs A, McArgs M, const VT *__restrict__ X) {
   // every product and sum is rounded on its own (no fused multiply-add), so
   // the one-column form below and the NV-column form round a column's terms
   // identically: a vector gets the same bits alone and in any block
@@ -378,9 +378,12 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
   // (position, canonical coefficient) per target; with one column the
   // target's contribution (re, im) instead
   __shared__ double tcr[BAL_CAP], tci[(CPLX || NV == 1) ? BAL_CAP : 1];
-  __shared__ uint32_t roff[BAL_TPB], rcnt[BAL_TPB];
-  __shared__ uint8_t rpop[BAL_TPB][FAST_MAXFAM * 4];
-  __shared__ ST rinv[BAL_TPB];
+  // per-row tables for the NV-lane phase 3; one column keeps them in the row
+  // thread's registers (6 instead of 4 workgroups per CU)
+  constexpr int NROW = NV == 1 ? 1 : BAL_TPB;
+  __shared__ uint32_t roff[NROW], rcnt[NROW];
+  __shared__ uint8_t rpop[NROW][FAST_MAXFAM * 4];
+  __shared__ ST rinv[NROW];
   __shared__ uint32_t wsum[BAL_TPB / 64];
   constexpr bool RUNS = (LT == 32 || LT == 1) && KIND != EDK_PROJECTED;
   __shared__ uint16_t dq[RUNS ? BAL_CAP : 1];
@@ -424,10 +427,12 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
       const int d = T.fam[f].d;
       uint32_t r = rotr_lt<LT>(s, A.L - d, A.L, A.mask);
       uint32_t nr = ~r & A.mask;
-      rpop[tid][f * 4 + 0] = (uint8_t)__popc(ns & nr);
-      rpop[tid][f * 4 + 1] = (uint8_t)__popc(ns & r);
-      rpop[tid][f * 4 + 2] = (uint8_t)__popc(s & nr);
-      rpop[tid][f * 4 + 3] = (uint8_t)__popc(s & r);
+      if constexpr (NV > 1) {
+        rpop[tid][f * 4 + 0] = (uint8_t)__popc(ns & nr);
+        rpop[tid][f * 4 + 1] = (uint8_t)__popc(ns & r);
+        rpop[tid][f * 4 + 2] = (uint8_t)__popc(s & nr);
+        rpop[tid][f * 4 + 3] = (uint8_t)__popc(s & r);
+      }
       for (int e = 0; e < T.fam[f].nent; e++) {
         const int flip = T.fam[f].ent[e].flip;
         uint32_t bits = ent_mask(T.fam[f].ent[e].locs, s, ns, r, A.mask);
@@ -446,9 +451,11 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
         }
       }
     }
-    roff[tid] = off;
-    rcnt[tid] = n;
-    rinv[tid] = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[r0 + tid]);
+    if constexpr (NV > 1) {
+      roff[tid] = off;
+      rcnt[tid] = n;
+      rinv[tid] = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[r0 + tid]);
+    }
   }
   __syncthreads();
   // phase 2: BAL_K targets per thread -> (position, canonical coefficient)
@@ -560,10 +567,17 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
     if (own && M.y[0]) {
       VT *yc = (VT *)M.y[0];
       const uint32_t q0 = off, q1 = off + n;
+      // the row's popcounts and 1/R_i again (the values the NV-lane form
+      // keeps in LDS)
+      const uint32_t ns = ~s & A.mask;
       ST dg = 0;
-      for (int f = 0; f < T.nfam; f++)
-        dg += (ST)(sD[0][f * 4 + 0] * rpop[tid][f * 4 + 0] + sD[0][f * 4 + 1] * rpop[tid][f * 4 + 1] +
-                   sD[0][f * 4 + 2] * rpop[tid][f * 4 + 2] + sD[0][f * 4 + 3] * rpop[tid][f * 4 + 3]);
+      for (int f = 0; f < T.nfam; f++) {
+        const uint32_t r = rotr_lt<LT>(s, A.L - T.fam[f].d, A.L, A.mask), nr = ~r & A.mask;
+        const uint8_t p0 = (uint8_t)__popc(ns & nr), p1 = (uint8_t)__popc(ns & r), p2 = (uint8_t)__popc(s & nr),
+                      p3 = (uint8_t)__popc(s & r);
+        dg += (ST)(sD[0][f * 4 + 0] * p0 + sD[0][f * 4 + 1] * p1 + sD[0][f * 4 + 2] * p2 + sD[0][f * 4 + 3] * p3);
+      }
+      const ST rinv0 = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[r0 + tid]);
       ST orr = 0, oi = 0;
       for (uint32_t q = q0; q < q1; q++) {
         if (tgt[q] == HASH_EMPTY) continue;
@@ -572,8 +586,8 @@ __global__ void __launch_bounds__(BAL_TPB) k_matvec_bal_mc(FastArgs A, McArgs M,
       }
       const VT xi = X[r0 + tid];
       VT out;
-      out.x = dg * (ST)xi.x + orr * rinv[tid];
-      out.y = dg * (ST)xi.y + oi * rinv[tid];
+      out.x = dg * (ST)xi.x + orr * rinv0;
+      out.y = dg * (ST)xi.y + oi * rinv0;
       yc[r0 + tid] = out;
     }
     return;
