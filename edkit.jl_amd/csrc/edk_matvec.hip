@@ -215,7 +215,7 @@ __device__ __forceinline__ uint32_t gt_block_scan(uint32_t v, uint32_t *total, u
 // coefficient in LDS and phase 3 gives each row NV lanes, one per column, that
 // walk the row's targets in order (each lookup serves all NV columns).
 template <typename VT, typename ST, int NV>
-__global__ void __launch_bounds__(GT_TPB) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
+__global__ void __launch_bounds__(GT_TPB, 6) k_matvec_gtile(MatvecArgs A, const uint32_t *__restrict__ chunk,
                                                          const VT *__restrict__ x, VT *__restrict__ y,
                                                          int accumulate, int64_t nb, int64_t col) {
   __shared__ uint32_t s_off[NV ? GT_TPB / GT_Q : 1], s_n[NV ? GT_TPB / GT_Q : 1];
