@@ -8,8 +8,10 @@ Cases cover the pruned walk (generator 0 maps the first row onto itself)
 and the full walk (generator 0 first in the list does not, or the pruning
 table is too large for LDS), real and complex phases, a third (spin-flip)
 generator, bonds listed in either orientation, four families (next-nearest
-neighbours with their own matrix) and repeated bonds; an operator with
-one-site terms keeps the generic kernel."""
+neighbours with their own matrix) and repeated bonds, keyword bases (one
+translation generator, a reflection, a spin flip) and a ring translation
+given as a permutation; an operator with one-site terms, and a basis
+without a fixed charge, keep the generic kernel."""
 import numpy as np
 import pytest
 
@@ -123,4 +125,62 @@ def test_one_site_terms_keep_the_generic_kernel(gpu):
     x = rand_state(B.D, 7)
     yo = orc.Op(ob, H2.flatten()) @ x
     y = H2 @ x
+    assert np.linalg.norm(y - yo) <= 1e-12 * np.linalg.norm(yo)
+
+
+def keyword_pair(edk, L, **kw):
+    """basis(...) with k/p/z keywords (translation, reflection and spin-flip
+    generators of the Abelian walk) and its oracle twin."""
+    B = edk.basis(L=L, **kw)
+    ob = orc.Basis("abelian", L, N=kw.get("N"), k=kw.get("k") or 0, p=kw.get("p") or 0, z=kw.get("z") or 0,
+                   has_k="k" in kw, has_p="p" in kw, has_z="z" in kw)
+    return B, ob
+
+
+@pytest.mark.parametrize("kw", [dict(N=7, k=2), dict(N=7, k=0, p=1, z=1), dict(N=7, k=7, p=-1, z=-1),
+                                dict(N=6, k=6, p=1)])
+def test_keyword_abelian_ring(gpu, kw):
+    """One generator (a translation: unpruned walk), a reflection (bit
+    reversal) and a spin flip (inversion) through the two-site kernel, on the
+    XXZ and J1-J2 rings.  k and p together only at k = 0 or pi, the sectors
+    where translation and reflection commute (the reference's notes on
+    basis(...), AbelianBasis.jl:749-750)."""
+    import edkit_hip as edk
+    from tests.models import j1j2, xxz
+    L = 14 if kw["N"] == 7 else 12
+    B, ob = keyword_pair(edk, L, **kw)
+    check_paths(edk, B, ob, xxz(B, 0.7), 500 + L)
+    check_paths(edk, B, ob, j1j2(B, 0.35), 600 + L)
+
+
+def test_custom_translation_single_generator(gpu):
+    """A ring translation given as a site permutation (the only generator)."""
+    import edkit_hip as edk
+    L, N, k = 14, 7, 3
+    t = [(i + 1) % L + 1 for i in range(L)]
+    sym = [(t, k)]
+    B = edk.basis(L=L, N=N, symmetries=sym)
+    ob = orc.Basis("abelian", L, N=N, symmetries=sym)
+    from edkit_hip.operator import operator
+    H = operator([heis(0.6)] * L, [[i + 1, (i + 1) % L + 1] for i in range(L)], B)
+    check_paths(edk, B, ob, H, 700)
+
+
+def test_without_fixed_charge_keeps_the_generic_kernel(gpu):
+    """Without N the representatives' popcounts differ: no rank directory,
+    so the generic kernel runs (and still matches the oracle)."""
+    import edkit_hip as edk
+    from edkit_hip import _lib
+    from edkit_hip.matvec import device_operator
+    Lx, Ly = 4, 3
+    tx, ty = torus_translations(Lx, Ly)
+    sym = [(tx, 1), (ty, 0)]
+    B = edk.basis(L=Lx * Ly, symmetries=sym)
+    ob = orc.Basis("abelian", Lx * Ly, symmetries=sym)
+    H = heisenberg_2d(B, Lx, Ly, delta=0.8)
+    with pytest.raises(edk.EdkError):
+        device_operator(H).set_variant(np.complex128, 1, _lib.EDK_VARIANT_TILE)
+    x = rand_state(B.D, 9)
+    yo = orc.Op(ob, H.flatten()) @ x
+    y = H @ x
     assert np.linalg.norm(y - yo) <= 1e-12 * np.linalg.norm(yo)
