@@ -168,57 +168,81 @@ __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, con
     }
   }
   __syncthreads();
-  // phase 3: NV threads per row, targets summed in generator order per column
-  for (uint32_t idx = tid; idx < nrow * NV; idx += BAL_TPB) {
-    const uint32_t row = idx / NV, c = idx % NV;
+  // phase 3: NV / CPL threads per row, each summing CPL columns' targets in
+  // generator order.  A block of 8 takes two columns a thread, so a chunk's
+  // row-column pairs fit one pass of the block (with one column a thread, a
+  // chunk of more than 32 rows needs a second pass that the first wave runs
+  // alone, lengthening the block's chain of gathers)
+  constexpr int CPL = NV == 8 ? 2 : 1;
+  constexpr int LPR = NV / CPL;
+  constexpr int GF = (CPL == 2 && sizeof(VT) > 8) ? 4 : 8;  // targets' gathers in flight
+  for (uint32_t idx = tid; idx < nrow * LPR; idx += BAL_TPB) {
+    const uint32_t row = idx / LPR, c = (idx % LPR) * CPL;
     const uint32_t q0 = roff[row], q1 = q0 + rcnt[row];
-    ST orr = 0, oi = 0;
-    // four gathers in flight per thread; an absent target reads row 0 and
-    // adds nothing (terms are added in generator order either way)
-    auto term = [&](uint32_t q, const VT &xj) {
+    ST orr[CPL], oi[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+      orr[k] = 0;
+      oi[k] = 0;
+    }
+    // an absent target reads row 0 and adds nothing (terms are added in
+    // generator order either way)
+    auto term = [&](uint32_t q, const VT (&xj)[CPL]) {
       const bool hit = tgt[q] != HASH_EMPTY;
-      if (CPLX) {
-        orr += hit ? tcr[q] * (ST)xj.x - tci[q] * (ST)xj.y : (ST)0;
-        oi += hit ? tcr[q] * (ST)xj.y + tci[q] * (ST)xj.x : (ST)0;
-      } else {
-        orr += hit ? tcr[q] * (ST)xj.x : (ST)0;
-        oi += hit ? tcr[q] * (ST)xj.y : (ST)0;
+#pragma unroll
+      for (int k = 0; k < CPL; k++) {
+        if (CPLX) {
+          orr[k] += hit ? tcr[q] * (ST)xj[k].x - tci[q] * (ST)xj[k].y : (ST)0;
+          oi[k] += hit ? tcr[q] * (ST)xj[k].y + tci[q] * (ST)xj[k].x : (ST)0;
+        } else {
+          orr[k] += hit ? tcr[q] * (ST)xj[k].x : (ST)0;
+          oi[k] += hit ? tcr[q] * (ST)xj[k].y : (ST)0;
+        }
       }
     };
-    auto gather = [&](uint32_t q) {
+    auto gather = [&](uint32_t q, VT (&xj)[CPL]) {
       const uint32_t jj = tgt[q];
-      return x[(size_t)(jj != HASH_EMPTY ? jj : 0u) * NV + c];
+      const VT *xr = x + (size_t)(jj != HASH_EMPTY ? jj : 0u) * NV + c;
+#pragma unroll
+      for (int k = 0; k < CPL; k++) xj[k] = xr[k];
     };
     uint32_t q = q0;
-    for (; q + 8 <= q1; q += 8) {
-      VT xs[8];
+    for (; q + GF <= q1; q += GF) {
+      VT xs[GF][CPL];
 #pragma unroll
-      for (int u = 0; u < 8; u++) xs[u] = gather(q + u);
+      for (int u = 0; u < GF; u++) gather(q + u, xs[u]);
 #pragma unroll
-      for (int u = 0; u < 8; u++) term(q + u, xs[u]);
+      for (int u = 0; u < GF; u++) term(q + u, xs[u]);
     }
     for (; q + 4 <= q1; q += 4) {
-      const VT x0 = gather(q), x1 = gather(q + 1), x2 = gather(q + 2), x3 = gather(q + 3);
-      term(q, x0);
-      term(q + 1, x1);
-      term(q + 2, x2);
-      term(q + 3, x3);
+      VT xs[4][CPL];
+#pragma unroll
+      for (int u = 0; u < 4; u++) gather(q + u, xs[u]);
+#pragma unroll
+      for (int u = 0; u < 4; u++) term(q + u, xs[u]);
     }
-    for (; q < q1; q++) term(q, gather(q));
-    const size_t i = (size_t)(r0 + row) * NV + c;
-    const VT xi = x[i];
-    const ST yr = rdg[row] * (ST)xi.x + orr * rinv[row];
-    const ST yi = rdg[row] * (ST)xi.y + oi * rinv[row];
-    VT out;
-    if (accumulate) {
-      const VT y0 = y[i];
-      out.x = y0.x + yr;
-      out.y = y0.y + yi;
-    } else {
-      out.x = yr;
-      out.y = yi;
+    for (; q < q1; q++) {
+      VT xs[CPL];
+      gather(q, xs);
+      term(q, xs);
     }
-    y[i] = out;
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+      const size_t i = (size_t)(r0 + row) * NV + c + k;
+      const VT xi = x[i];
+      const ST yr = rdg[row] * (ST)xi.x + orr[k] * rinv[row];
+      const ST yi = rdg[row] * (ST)xi.y + oi[k] * rinv[row];
+      VT out;
+      if (accumulate) {
+        const VT y0 = y[i];
+        out.x = y0.x + yr;
+        out.y = y0.y + yi;
+      } else {
+        out.x = yr;
+        out.y = yi;
+      }
+      y[i] = out;
+    }
   }
 }
 
