@@ -11,13 +11,15 @@ template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
 __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, const VT *__restrict__ x,
                                              VT *__restrict__ y, int accumulate) {
   constexpr bool CPLX = (PH == 2);
-  __shared__ double sR[2 * 64];
-  __shared__ double2 sP[PH != 0 ? 64 : 1];
+  // orbit lengths are at most 32 (32-bit keys): norms for both parities and
+  // one phase per rotation count
+  __shared__ double sR[2 * 32];
+  __shared__ double2 sP[PH != 0 ? 32 : 1];
   __shared__ double sV[FAST_MAXFAM * FAST_MAXENT];
   __shared__ uint32_t tgt[BAL_CAP];
   __shared__ uint8_t ent[BAL_CAP];
   __shared__ ST tcr[BAL_CAP], tci[CPLX ? BAL_CAP : 1];
-  __shared__ uint32_t roff[BAL_TPB], rcnt[BAL_TPB];
+  __shared__ uint32_t roff[BAL_TPB + 1];  // row r's targets: [roff[r], roff[r + 1])
   __shared__ ST rdg[BAL_TPB], rinv[BAL_TPB];
   __shared__ uint32_t wsum[BAL_TPB / 64];
   // run-length canonicaliser (unit cell 1, 16 <= L <= 32) with a block-wide
@@ -83,7 +85,7 @@ __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, con
       }
     }
     roff[tid] = off;
-    rcnt[tid] = n;
+    if ((uint32_t)tid == nrow - 1) roff[nrow] = total;
     rdg[tid] = dg;
     rinv[tid] = (KIND == EDK_PROJECTED) ? (ST)1 : (ST)(1.0 / A.Rrow[r0 + tid]);
   }
@@ -178,7 +180,7 @@ __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, con
   constexpr int GF = (CPL == 2 && sizeof(VT) > 8) ? 4 : 8;  // targets' gathers in flight
   for (uint32_t idx = tid; idx < nrow * LPR; idx += BAL_TPB) {
     const uint32_t row = idx / LPR, c = (idx % LPR) * CPL;
-    const uint32_t q0 = roff[row], q1 = q0 + rcnt[row];
+    const uint32_t q0 = roff[row], q1 = roff[row + 1];
     ST orr[CPL], oi[CPL];
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
