@@ -4,9 +4,10 @@
 
 // Block of NV vectors, row-interleaved (element (i, c) at x[i*NV + c]): the
 // same chunks and phases as k_matvec_bal, but phase 2 keeps each target's
-// (position, coefficient) in LDS, and phase 3 gives every row NV consecutive
-// threads, one per vector, so each representative lookup serves all NV
-// columns and the NV gathered values of a target form one contiguous segment.
+// (position, coefficient) in LDS, and phase 3 gives every row consecutive
+// threads of one or two columns each, so each representative lookup serves
+// all NV columns and the NV gathered values of a target form one contiguous
+// segment.
 template <int KIND, int PH, int LT, typename VT, typename ST, int NV>
 __device__ __forceinline__ void bal_blk_body(const FastArgs &A, uint32_t cb, const VT *__restrict__ x,
                                              VT *__restrict__ y, int accumulate) {
