@@ -23,6 +23,8 @@ BLOCK_CASES = [
     ("TranslationalBasis", 16, dict(N=7, k=3), j1j2),
     ("TranslationalBasis", 16, dict(N=8, k=3, a=2), xxz),
     ("ProjectedBasis", 14, dict(N=7), j1j2),
+    # at most 4 targets a row: the first chunk holds the full 256 rows
+    ("ProjectedBasis", 24, dict(N=2), xxz),
 ]
 
 
