@@ -199,6 +199,77 @@ __global__ void __launch_bounds__(RED_TPB) k_lz_step(const double2 *part, int nb
   }
 }
 
+// For small vectors, the device-driven steps fold the one-block reductions
+// into the kernels that consume them: every block sums the partials in
+// k_reduce_part's order and tree (so each holds the same doubles), block 0
+// publishes the result.  Saves a launch per reduction where the blocks'
+// repeated reads of the partials cost less than a launch.
+//
+// w -= Re(alpha_j) v_j with alpha_j the sum of the partials, written to *out
+template <typename VT>
+__global__ void __launch_bounds__(RED_TPB) k_axpy_re_red(VT *y, const VT *x, const double2 *part, int nb,
+                                                         double2 *out, double sign, int64_t n) {
+  __shared__ double2 sa;
+  double2 acc = make_double2(0, 0);
+  for (int b = threadIdx.x; b < nb; b += RED_TPB) {
+    acc.x += part[b].x;
+    acc.y += part[b].y;
+  }
+  const double2 r = block_sum2(acc);
+  if (threadIdx.x == 0) {
+    sa = r;
+    if (blockIdx.x == 0) out[0] = r;
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x;
+  if (i >= n) return;
+  const double a = sign * sa.x;
+  double2 u = ld(y, i), b = ld(x, i);
+  u.x += a * b.x - 0.0 * b.y;
+  u.y += a * b.y + 0.0 * b.x;
+  st(y, i, u);
+}
+
+// k_lz_step and k_scale_dev in one launch: every block takes beta_j from the
+// norm partials as k_lz_step does, block 0 records the step's scalars and
+// decisions, and (scale != 0) the blocks write v_{j+1} = w / beta_j
+template <typename VT>
+__global__ void __launch_bounds__(RED_TPB) k_lz_step_scale(const double2 *part, int nb, const double2 *s, int j,
+                                                           double2 *lz, VT *vn, const VT *w, int64_t n,
+                                                           int scale) {
+  __shared__ double sinv;
+  double2 acc = make_double2(0, 0);
+  for (int b = threadIdx.x; b < nb; b += RED_TPB) {
+    acc.x += part[b].x;
+    acc.y += part[b].y;
+  }
+  const double2 r = block_sum2(acc);
+  if (threadIdx.x == 0) {
+    const double w2 = r.x;
+    const double bj = sqrt(w2);
+    sinv = 1.0 / bj;
+    if (blockIdx.x == 0) {
+      double hh = 0.0;
+      for (int c = 0; c < j; c++) hh += s[c].x * s[c].x + s[c].y * s[c].y;
+      const bool second = w2 < 0.5 * (w2 + hh);
+      const double aj = lz[j - 1].x;
+      lz[LZ_M + j - 1] = make_double2(bj, second ? 1.0 : 0.0);
+      lz[2 * LZ_M] = make_double2(1.0 / bj, 0.0);
+      const double sq = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+      if (lz[2 * LZ_M + 1].x == 0.0) {
+        if (second) lz[2 * LZ_M + 1] = make_double2((double)j, 1.0);
+        else if (bj <= sq * fmax(1.0, fabs(aj))) lz[2 * LZ_M + 1] = make_double2((double)j, 2.0);
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * RED_TPB + threadIdx.x;
+  if (!scale || i >= n) return;
+  const double sc = sinv;
+  double2 b = ld(w, i);
+  st(vn, i, make_double2(b.x * sc, b.y * sc));
+}
+
 // start of a device-driven extension from step m0: no step settled yet, and
 // beta_{m0} (from the host's array) where step m0 + 1 reads it
 __global__ void k_lz_start(double2 *lz, int m0, double beta_m0) {
@@ -895,6 +966,10 @@ static int lanczos_extend_dev(edk_op *op, KrylovWork &K, void *V, int64_t ldv, v
   const int64_t n = K.n;
   const unsigned g = (unsigned)((n + 255) / 256);
   const int dtype = K.dtype;
+  // fold the one-block reductions into their consumers while every block's
+  // read of the partials stays small (e.g. 441 blocks x 441 partials at
+  // D = 112 720)
+  const bool fold = (int64_t)g * K.nblk <= (1 << 18);
   k_lz_start<<<1, 1, 0, s>>>(lz, m0, m0 >= 1 ? beta[m0 - 1] : 0.0);
   for (int j = m0 + 1; j <= m_target; j++) {
     char *vj = (char *)V + (size_t)(j - 1) * ldv * es;
@@ -908,15 +983,24 @@ static int lanczos_extend_dev(edk_op *op, KrylovWork &K, void *V, int64_t ldv, v
     } else {
       DISPATCH(VT, k_dot_part<VT><<<K.nblk, RED_TPB, 0, s>>>((const VT *)vj, (const VT *)w, n, K.d_part));
     }
-    k_reduce_part<<<1, RED_TPB, 0, s>>>(K.d_part, K.nblk, lz + (j - 1));
-    DISPATCH(VT, k_axpy_re_dev<VT><<<g, 256, 0, s>>>((VT *)w, (const VT *)vj, lz + (j - 1), -1.0, n));
+    if (fold) {
+      DISPATCH(VT, k_axpy_re_red<VT><<<g, RED_TPB, 0, s>>>((VT *)w, (const VT *)vj, K.d_part, K.nblk, lz + (j - 1),
+                                                           -1.0, n));
+    } else {
+      k_reduce_part<<<1, RED_TPB, 0, s>>>(K.d_part, K.nblk, lz + (j - 1));
+      DISPATCH(VT, k_axpy_re_dev<VT><<<g, 256, 0, s>>>((VT *)w, (const VT *)vj, lz + (j - 1), -1.0, n));
+    }
     // one sweep, then ||w||^2 and the step's scalars
     if (int rc = K.cgs_sweep(V, ldv, j, w)) return -rc;
     DISPATCH(VT, k_dot_part<VT><<<K.nblk, RED_TPB, 0, s>>>((const VT *)w, (const VT *)nullptr, n, K.d_part));
-    k_lz_step<<<1, RED_TPB, 0, s>>>(K.d_part, K.nblk, K.d_s, j, lz);
-    if (j + 1 <= ncols) {
-      char *vn = (char *)V + (size_t)j * ldv * es;
-      DISPATCH(VT, k_scale_dev<VT><<<g, 256, 0, s>>>((VT *)vn, (const VT *)w, lz + 2 * LZ_M, n));
+    char *vn = (char *)V + (size_t)j * ldv * es;
+    const bool next = j + 1 <= ncols;
+    if (fold) {
+      DISPATCH(VT, k_lz_step_scale<VT><<<next ? g : 1u, RED_TPB, 0, s>>>(K.d_part, K.nblk, K.d_s, j, lz, (VT *)vn,
+                                                                         (const VT *)w, n, next ? 1 : 0));
+    } else {
+      k_lz_step<<<1, RED_TPB, 0, s>>>(K.d_part, K.nblk, K.d_s, j, lz);
+      if (next) DISPATCH(VT, k_scale_dev<VT><<<g, 256, 0, s>>>((VT *)vn, (const VT *)w, lz + 2 * LZ_M, n));
     }
     if (hipGetLastError() != hipSuccess) return -edk_fail(EDK_ERR_HIP, "Lanczos step launch");
   }
