@@ -205,7 +205,7 @@ __device__ __forceinline__ bool fast_canon(const FastArgs &A, uint32_t s, uint32
 // orbit period (a power of two dividing 32) is read off the same rotations.
 template <int KIND, int PH>
 __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint32_t &rep, double &cr,
-                                             double &ci) {
+                                             double &ci, int *ri = nullptr) {
   if (KIND == EDK_PROJECTED) {
     rep = s;
     cr = 1.0;
@@ -229,6 +229,7 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
     rep = minA;
     if (((int64_t)period * A.kp) % 32) return false;  // check_momentum
     double R = A.Rtab[period - 1];
+    if (ri) *ri = period - 1;
     if (PH != 0) {
       int M = 0;
       bool found = (s == minA);
@@ -260,6 +261,7 @@ __device__ __forceinline__ bool fast_canon32(const FastArgs &A, uint32_t s, uint
   minB = min(minB, __builtin_amdgcn_alignbit(cur, cur, 1));
   bool qp = (minA == minB);
   double R = A.Rtab[(qp ? 32 : 0) + period - 1];
+  if (ri) *ri = (qp ? 32 : 0) + period - 1;
   double sgn = 1.0;
   if (PH != 0) {
     // first k with rot^k(s) == minA, resp. reverse(rot^k(s)) == minB
@@ -336,7 +338,9 @@ __device__ __forceinline__ bool canon32_runs(uint32_t s, uint32_t &minA, uint32_
 // word take the smaller count).  Returns 1 with rep and the factor
 // conj(phase) * R (cr, ci) exactly as fast_canon / fast_canon32 compute it,
 // 0 when the representative cannot be in the sector, -1 when the word has
-// more than two longest runs (the caller runs the full search).
+// more than two longest runs (the caller runs the full search).  Given ri,
+// these canonicalisers (and fast_canon32, fast_canon_w) also store the index
+// of R in the norm table there.
 template <int LT>
 __device__ __forceinline__ uint32_t rotr_w(uint32_t x, uint32_t m, int L, uint32_t mask) {
   if (LT == 32) return __builtin_amdgcn_alignbit(x, x, m);
@@ -357,7 +361,8 @@ __device__ __forceinline__ uint32_t rotl_w(uint32_t x, uint32_t m, int L, uint32
 // windows order as their words do, and bitwise operations keep windows
 // periodic.  Same candidates, rotation counts and results as canon_runs.
 template <int KIND, int PH>
-__device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci) {
+__device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci,
+                                            int *ri = nullptr) {
   const uint32_t L = (uint32_t)B.L, c = 32u - L;
   const uint32_t W = (s << c) | (s >> (L - c)), Wc = W << c;
 #define EDK_WROTL(X, XC, M) __builtin_amdgcn_alignbit((X), (XC), 32u - (M))
@@ -400,6 +405,7 @@ __device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint3
     // check_momentum: period * k' divisible by L, period in {L, L/2}
     if (period != L && (B.kp & 1)) return 0;
     const double R = B.Rtab[period - 1];
+    if (ri) *ri = period - 1;
     if (PH != 0) {
       const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
       const double2 cc = B.ph[Ma];
@@ -426,6 +432,7 @@ __device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint3
   const uint32_t minB = min(vb1, vb2);
   const bool qp = (minA == minB);
   const double R = B.Rtab[(qp ? L : 0) + period - 1];
+  if (ri) *ri = (qp ? L : 0) + period - 1;
   double sgn = 1.0;
   if (PH != 0) {
     const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
@@ -447,7 +454,7 @@ __device__ __forceinline__ int canon_runs_w(const FastArgs &B, uint32_t s, uint3
 // and the first count that maps the word onto itself is its period.
 template <int KIND, int PH>
 __device__ __forceinline__ bool fast_canon_w(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr,
-                                             double &ci) {
+                                             double &ci, int *ri = nullptr) {
   const uint32_t L = (uint32_t)B.L, c = 32u - L;
   const uint32_t W = (s << c) | (s >> (L - c)), Wc = W << c;
   uint32_t minA = W, Ma = 0, period = L;
@@ -461,6 +468,7 @@ __device__ __forceinline__ bool fast_canon_w(const FastArgs &B, uint32_t s, uint
     rep = minA >> c;
     if (((int64_t)period * B.kp) % (int64_t)L) return false;  // check_momentum
     const double R = B.Rtab[period - 1];
+    if (ri) *ri = period - 1;
     if (PH != 0) {
       const double2 cc = B.ph[Ma];
       cr = cc.x * R;
@@ -482,6 +490,7 @@ __device__ __forceinline__ bool fast_canon_w(const FastArgs &B, uint32_t s, uint
   }
   const bool qp = (minA == minB);
   const double R = B.Rtab[(qp ? L : 0) + period - 1];
+  if (ri) *ri = (qp ? L : 0) + period - 1;
   double sgn = 1.0;
   if (PH != 0) {
     if (minB < minA) sgn = (double)B.P * ((B.kp == 1 || (Mb & 1) == 0) ? 1.0 : -1.0);
@@ -494,8 +503,9 @@ __device__ __forceinline__ bool fast_canon_w(const FastArgs &B, uint32_t s, uint
 }
 
 template <int KIND, int PH, int LT>
-__device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci) {
-  if constexpr (LT == 1) return canon_runs_w<KIND, PH>(B, s, rep, cr, ci);
+__device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_t &rep, double &cr, double &ci,
+                                          int *ri = nullptr) {
+  if constexpr (LT == 1) return canon_runs_w<KIND, PH>(B, s, rep, cr, ci, ri);
   const int L = (LT == 32) ? 32 : B.L;
   const uint32_t mask = (LT == 32) ? 0xffffffffu : B.mask;
   const uint32_t z1 = ~s & mask;
@@ -533,6 +543,7 @@ __device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_
     // check_momentum: period * k' divisible by L, period in {L, L/2}
     if (period != L && (B.kp & 1)) return 0;
     const double R = B.Rtab[period - 1];
+    if (ri) *ri = period - 1;
     if (PH != 0) {
       const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;
       const double2 c = B.ph[Ma];
@@ -556,6 +567,7 @@ __device__ __forceinline__ int canon_runs(const FastArgs &B, uint32_t s, uint32_
   const uint32_t minB = min(vb1, vb2);
   const bool qp = (minA == minB);
   const double R = B.Rtab[(qp ? L : 0) + period - 1];
+  if (ri) *ri = (qp ? L : 0) + period - 1;
   double sgn = 1.0;
   if (PH != 0) {
     const uint32_t Ma = (va1 == minA && (va2 != minA || ia1 < ia2)) ? ia1 : ia2;

@@ -20,26 +20,36 @@
 #include "edk_fast.h"
 
 // target meta word: entry (bits 0-5), first site's bit (6-10), family
-// distance (11-12), exchange rule (13), absent (14), row in the chunk (16-23)
+// distance (11-12), exchange rule (13), absent (14), row in the chunk (16-23),
+// the norm-table index of the representative's factor (24-29) and its sign (30)
 #define RKM_XR 0x2000u
 #define RKM_ABSENT 0x4000u
+#define RKM_NEG 0x40000000u
 
 template <int KIND, int PH, int LT, typename VT, typename ST>
-__global__ void __launch_bounds__(BAL_TPB, 6)
+// 8 waves per SIMD (64 VGPRs); the real-sign L = 32 form spills there and keeps 6
+__global__ void __launch_bounds__(BAL_TPB, (PH == 1 && LT == 32) ? 6 : 8)
     k_matvec_rk(FastArgs A, const VT *__restrict__ x, VT *__restrict__ y, int accumulate) {
   static_assert(PH != 2, "real phases only");
   static_assert(LT == 32 || LT == 1, "run-length canonicalisers only");
-  // per target: word (then representative, then rank), meta word and
-  // coefficient, as three arrays (conflict-free 4-byte accesses); after the
-  // last reads of phase 2c the contributions overlay them
-  constexpr int UB = BAL_CAP * (int)(sizeof(VT) > 8 + sizeof(ST) ? sizeof(VT) : 8 + sizeof(ST));
-  __shared__ double sR[2 * 64];
+  // per target: word (then representative, then rank) and meta word, as two
+  // arrays (conflict-free 4-byte accesses).  A real coefficient is the
+  // entry's value times +-R from the norm table, so the meta word carries the
+  // table index and sign instead of an 8-byte coefficient.  After the last
+  // reads of phase 2c the contributions overlay the arrays, in NH passes of
+  // HCAP targets (ComplexF64: two passes of 512).  The arrays stay at 8 KB:
+  // 18 KB of LDS for the workgroup, 8 workgroups per CU.
+  constexpr int NH = sizeof(VT) > 8 ? 2 : 1;
+  constexpr int HCAP = BAL_CAP / NH;
+  static_assert(HCAP == BAL_TPB * (BAL_K / NH) && HCAP * (int)sizeof(VT) <= BAL_CAP * 8,
+                "a pass holds whole target slots and fits the target arrays");
+  // orbit lengths are at most 32 (32-bit keys): norms for both parities
+  __shared__ double sR[2 * 32];
   __shared__ double sV[FAST_MAXFAM * FAST_MAXENT];
   __shared__ uint32_t sT[RK_NT];
-  __shared__ __align__(16) unsigned char ub[UB];
-  uint32_t *const tw = reinterpret_cast<uint32_t *>(ub);
-  uint32_t *const tm = tw + BAL_CAP;
-  ST *const tc = reinterpret_cast<ST *>(tm + BAL_CAP);
+  __shared__ __align__(16) uint32_t ub[2 * BAL_CAP];
+  uint32_t *const tw = ub;
+  uint32_t *const tm = ub + BAL_CAP;
   VT *const contr = reinterpret_cast<VT *>(ub);
   __shared__ uint16_t rq[BAL_CAP];  // brute-force queue from the front, rank queue from the back
   __shared__ uint32_t rrow[BAL_TPB];
@@ -121,9 +131,10 @@ __global__ void __launch_bounds__(BAL_TPB, 6)
       const uint32_t t = tw[j], m = tm[j];
       uint32_t r = 0;
       double cr = 0.0, ci = 0.0;
-      const int st = canon_runs<KIND, PH, LT>(B, t, r, cr, ci);
+      int ri = 0;
+      const int st = canon_runs<KIND, PH, LT>(B, t, r, cr, ci, &ri);
       if (st > 0) {
-        tc[j] = (ST)(sV[m & 63u] * cr);
+        tm[j] = m | ((uint32_t)ri << 24) | (cr < 0.0 ? RKM_NEG : 0u);
         const int p = (m >> 6) & 31, d = (m >> 11) & 3, lo2 = p - d;
         if ((m & RKM_XR) && r == t && lo2 >= A.rlo && p < A.rlo + A.rbits) {
           const uint32_t sw = t ^ ((1u << p) | (1u << lo2));  // the row's word
@@ -162,9 +173,11 @@ __global__ void __launch_bounds__(BAL_TPB, 6)
       bool okb = true;
       if (brute) {
         double cr = 0.0, ci = 0.0;
+        int ri = 0;
         const uint32_t t = r;
-        okb = (LT == 32) ? fast_canon32<KIND, PH>(B, t, r, cr, ci) : fast_canon_w<KIND, PH>(B, t, r, cr, ci);
-        if (okb) tc[j] = (ST)(sV[tm[j] & 63u] * cr);
+        okb = (LT == 32) ? fast_canon32<KIND, PH>(B, t, r, cr, ci, &ri)
+                         : fast_canon_w<KIND, PH>(B, t, r, cr, ci, &ri);
+        if (okb) tm[j] = (tm[j] & 0x00ffffffu) | ((uint32_t)ri << 24) | (cr < 0.0 ? RKM_NEG : 0u);
       }
       okb = okb && rank_domain(A, r);
       if (okb) tw[j] = rank_full(sT, (r & A.dmask) >> A.rlo, A.rng);
@@ -184,8 +197,11 @@ __global__ void __launch_bounds__(BAL_TPB, 6)
     cf[q] = 0;
     if (j < total) {
       rk[q] = tw[j];
-      cf[q] = tc[j];
-      ok[q] = !(tm[j] & RKM_ABSENT);
+      const uint32_t m = tm[j];
+      ok[q] = !(m & RKM_ABSENT);
+      // the coefficient as phase 2 formed it: value * (sign * R)
+      const double R = sR[(m >> 24) & 63u];
+      cf[q] = (ST)(sV[m & 63u] * ((m & RKM_NEG) ? -R : R));
     }
   }
   rank_lookup_many<BAL_K>(A, rk, ok, jj);
@@ -194,47 +210,57 @@ __global__ void __launch_bounds__(BAL_TPB, 6)
   for (int q = 0; q < BAL_K; q++) xg[q] = x[jj[q] != HASH_EMPTY ? jj[q] : 0u];
   // every thread has read its targets before any contribution overlays them
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < BAL_K; q++) {
-    const uint32_t j = tid + BAL_TPB * q;
-    const bool hit = jj[q] != HASH_EMPTY;
-    VT cv;
-    cv.x = hit ? cf[q] * (ST)xg[q].x : (ST)0;
-    cv.y = hit ? cf[q] * (ST)xg[q].y : (ST)0;
-    if (j < total) contr[j] = cv;
-  }
-  __syncthreads();
-  // phase 3
+  // phase 3: the diagonal, then the row's contributions in generator order,
+  // one pass of HCAP target slots at a time (the sums run on across passes)
+  const uint32_t qe = off + n;
+  uint32_t qq = off;
+  ST dg = 0, orr = 0, oi = 0;
   if (own) {
-    const uint32_t i = r0 + tid;
     const uint32_t ns = ~s & A.mask;
-    ST dg = 0;
     for (int f = 0; f < T.nfam; f++) {
       const uint32_t r = rotr_lt<LT>(s, A.L - T.fam[f].d, A.L, A.mask);
       const uint32_t nr = ~r & A.mask;
       dg += (ST)(T.fam[f].diag[0] * __popc(ns & nr) + T.fam[f].diag[1] * __popc(ns & r) +
                  T.fam[f].diag[2] * __popc(s & nr) + T.fam[f].diag[3] * __popc(s & r));
     }
-    const VT *cs = contr;
-    ST orr = 0, oi = 0;
-    uint32_t q = off;
-    const uint32_t qe = off + n;
-    for (; q + 4 <= qe; q += 4) {
-      const VT c0 = cs[q], c1 = cs[q + 1], c2 = cs[q + 2], c3 = cs[q + 3];
-      orr += c0.x;
-      oi += c0.y;
-      orr += c1.x;
-      oi += c1.y;
-      orr += c2.x;
-      oi += c2.y;
-      orr += c3.x;
-      oi += c3.y;
+  }
+#pragma unroll
+  for (int h = 0; h < NH; h++) {
+#pragma unroll
+    for (int q = h * (BAL_K / NH); q < (h + 1) * (BAL_K / NH); q++) {
+      const uint32_t j = tid + BAL_TPB * q;
+      const bool hit = jj[q] != HASH_EMPTY;
+      VT cv;
+      cv.x = hit ? cf[q] * (ST)xg[q].x : (ST)0;
+      cv.y = hit ? cf[q] * (ST)xg[q].y : (ST)0;
+      if (j < total) contr[j - h * HCAP] = cv;
     }
-    for (; q < qe; q++) {
-      const VT cv = cs[q];
-      orr += cv.x;
-      oi += cv.y;
+    __syncthreads();
+    if (own) {
+      const uint32_t he = min(qe, (uint32_t)((h + 1) * HCAP));
+      const uint32_t hb = (uint32_t)(h * HCAP);
+      for (; qq + 4 <= he; qq += 4) {
+        const VT c0 = contr[qq - hb], c1 = contr[qq + 1 - hb], c2 = contr[qq + 2 - hb], c3 = contr[qq + 3 - hb];
+        orr += c0.x;
+        oi += c0.y;
+        orr += c1.x;
+        oi += c1.y;
+        orr += c2.x;
+        oi += c2.y;
+        orr += c3.x;
+        oi += c3.y;
+      }
+      for (; qq < he; qq++) {
+        const VT cv = contr[qq - hb];
+        orr += cv.x;
+        oi += cv.y;
+      }
     }
+    // the next pass overwrites the contributions
+    if (h + 1 < NH) __syncthreads();
+  }
+  if (own) {
+    const uint32_t i = r0 + tid;
     const ST yr = dg * (ST)xi.x + orr * invR;
     const ST yi = dg * (ST)xi.y + oi * invR;
     VT out;
