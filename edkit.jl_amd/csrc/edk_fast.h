@@ -777,11 +777,15 @@ __device__ __forceinline__ uint32_t rank_full(const uint32_t *T, uint32_t w, int
   return r;
 }
 
-// C(n, k) for n < 32, k <= 17 from the group table (a single set bit)
+// C(n, k) for n < 32, k <= 17 from the group table (a single set bit); the
+// index arithmetic stays in full-rate 24-bit multiplies (32-bit ones issue at
+// a quarter of the rate, and the rank kernel is bound by VALU issue)
 __device__ __forceinline__ uint32_t rank_binom(const uint32_t *T, int n, int k) {
-  const int g = (n * 11) >> 5;  // n / 3 for n < 32
-  const int idx = k > 0 ? ((g * RK_NCNT + k - 1) * 8 + (1 << (n - 3 * g))) : 0;
-  return k > 0 ? T[idx] : 1u;
+  const uint32_t un = (uint32_t)n;
+  const uint32_t g = __umul24(un, 11u) >> 5;  // n / 3 for n < 32
+  const uint32_t b = un - __umul24(g, 3u);
+  const uint32_t idx = __umul24(g, (uint32_t)(RK_NCNT * 8)) + ((uint32_t)(k - 1) << 3) + (1u << b);
+  return k > 0 ? T[k > 0 ? idx : 0u] : 1u;
 }
 
 // colex terms of the set bits of a window v of at most 3 bits at domain
