@@ -89,7 +89,7 @@ struct FastArgs {
   // rank directory (null when it does not apply): every representative is a
   // word with the bits of fix1 set, those of fix0 clear and rn of the rbits
   // bits in dmask (from bit rlo up) set; its colex rank among such words
-  // indexes granules of 96 ranks {representatives below the granule, bitmap}
+  // indexes granules of 64 ranks {representatives below the granule, bitmap}
   const uint4 *gran;
   uint32_t ngran;
   const uint32_t *rtab;   // 3-bit group table of the colex rank (RK_NT entries)
@@ -750,9 +750,11 @@ __device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint
 // Rank directory.  Representatives with a fixed number of set bits are
 // numbered by their colex rank, which for words of one popcount is their
 // numeric order: with p_1 < p_2 < ... the positions of the set domain bits,
-// rank = sum_i C(p_i, i).  A granule covers 96 consecutive ranks: the number
-// of representatives of lower rank and a 96-bit membership bitmap, so a
-// lookup is one 16-byte load at rank / 96.  The representatives cluster in
+// rank = sum_i C(p_i, i).  A granule covers 64 consecutive ranks: the number
+// of representatives of lower rank and a 64-bit membership bitmap in a
+// 16-byte slot, so a lookup is one 16-byte load at rank / 64 (a shift: 96
+// ranks a granule fill the slot but take quarter-rate multiplies to split a
+// rank, and the rank kernel is bound by VALU issue).  The representatives cluster in
 // rank (L = 32, N = 16: 9.4 M of them fill 13 % of the granules below the
 // largest rank), so the directory's hot part is a few MB against the 56 MB of
 // a key table with a bucket directory.
@@ -764,7 +766,7 @@ __device__ __forceinline__ void region_lookup_many(const FastArgs &A, const uint
 #define RK_NCNT 17                   // c = 0 .. 16 (at most 16 domain bits set)
 #define RK_NGRP 11                   // 3-bit groups of a 32-bit domain
 #define RK_NT (RK_NGRP * RK_NCNT * 8)
-#define RK_GRAN 96u                  // ranks per granule
+#define RK_GRAN 64u                  // ranks per granule
 
 // colex rank of the domain word w (ng groups)
 __device__ __forceinline__ uint32_t rank_full(const uint32_t *T, uint32_t w, int ng) {
@@ -811,6 +813,7 @@ __device__ __forceinline__ bool rank_domain(const FastArgs &A, uint32_t r) {
 template <int K>
 __device__ __forceinline__ void rank_lookup_many(const FastArgs &A, const uint32_t (&rk)[K], const bool (&ok)[K],
                                                  uint32_t (&jj)[K]) {
+  static_assert(RK_GRAN == 64u, "granules of two 32-bit bitmap words");
   uint4 g[K];
   bool in[K];
 #pragma unroll
@@ -821,11 +824,10 @@ __device__ __forceinline__ void rank_lookup_many(const FastArgs &A, const uint32
   }
 #pragma unroll
   for (int q = 0; q < K; q++) {
-    const uint32_t o = rk[q] % RK_GRAN;
-    const uint32_t wsel = o < 32u ? g[q].y : (o < 64u ? g[q].z : g[q].w);
-    const uint32_t bit = o & 31u;
-    const uint32_t below = (o >= 32u ? __popc(g[q].y) : 0u) + (o >= 64u ? __popc(g[q].z) : 0u) +
-                           __popc(wsel & ((1u << bit) - 1u));
+    const uint32_t o = rk[q] % RK_GRAN, bit = o & 31u;
+    const bool hi = o >= 32u;
+    const uint32_t wsel = hi ? g[q].z : g[q].y;
+    const uint32_t below = (hi ? __popc(g[q].y) : 0u) + __popc(wsel & ((1u << bit) - 1u));
     const bool hit = in[q] && ((wsel >> bit) & 1u);
     jj[q] = (g[q].x + below) | (hit ? 0u : HASH_EMPTY);
   }

@@ -408,7 +408,8 @@ __global__ void k_rank_keys(const uint32_t *keys, uint32_t D, const uint32_t *T,
   if (i < D) rk[i] = rank_full(T, (keys[i] & dmask) >> rlo, ng);
 }
 
-// granule g: representatives of rank below 96 g, membership of 96 g .. 96 g + 95
+// granule g: representatives of rank below 64 g, membership of 64 g .. 64 g + 63
+// (bitmap words 0 and 1; the slot's last word stays 0)
 __global__ void k_gran_build(const uint32_t *rk, uint32_t D, uint32_t ngran, uint4 *gran) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ngran) return;
