@@ -88,6 +88,7 @@ struct PermTab {
   PermGen gen[3];
   int pbits;                          // pruned walk's table: 2^pbits entries (0: the full walk)
   uint32_t rcp0;                      // ceil(2^16 / order of generator 0): e / n0 = (e rcp0) >> 16 for e < 64
+  uint32_t rcp1;                      // the same for generator 1 (an element's exponents without a division)
   PowGen pow0[PM_MAXPOW];             // generator 0's powers (pruned walk)
   double R[PM_MAXGRPORD];             // norm by stabiliser order
   double2 ph[3 * PM_MAXORD];          // phase of generator g's m-th power
@@ -218,11 +219,11 @@ __device__ __forceinline__ void perm_canon(int L, const Gen32 &g0, const Gen32 &
     while (cm) {
       const int e = __ffsll((unsigned long long)cm) - 1;
       cm &= cm - 1;
-      const int i = (int)(((uint32_t)e * rcp0) >> 16);
+      const int i = (int)(__umul24((uint32_t)e, rcp0) >> 16);  // e / n0 (e < 64: a 24-bit product)
       uint32_t c = co[0];
 #pragma unroll
       for (int k = 1; k < EDK_PRUNE_OUT; k++) c = i == k ? co[k] : c;
-      c = perm_apply_pow<NPW>(pw[e - i * n0], c);
+      c = perm_apply_pow<NPW>(pw[e - (int)__umul24((uint32_t)i, (uint32_t)n0)], c);
       if (cnt == 0 || c < best) {
         best = c;
         be = e;
@@ -302,6 +303,7 @@ __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
   const int tid = threadIdx.x;
   const Gen32 g0 = perm_gen(T, 0), g1 = perm_gen(T, 1), g2 = perm_gen(T, 2);
   const int n0 = T.gen[0].order, n1 = T.gen[1].order, n2 = T.gen[2].order;
+  const uint32_t rcp0 = T.rcp0, rcp1 = T.rcp1;
   {
     for (int q = tid; q < RK_NT; q += BAL_TPB) sT[q] = A.rtab[q];
     if (tid < PM_MAXFAM * FAST_MAXENT) {
@@ -374,7 +376,11 @@ __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
   // targets (its own slots only)
   auto finish = [&](uint32_t j, uint32_t rep, int be, int cnt) {
     const uint32_t m = tm[j];
-    const int m0 = be % n0, m1 = (be / n0) % n1, m2 = be / (n0 * n1);
+    // exponents of element be = m0 + n0 (m1 + n1 m2) by reciprocal 24-bit
+    // multiplies (be < 64; a runtime integer division costs quarter-rate
+    // multiplies in a kernel bound by VALU issue)
+    const uint32_t i1 = __umul24((uint32_t)be, rcp0) >> 16, i2 = __umul24(i1, rcp1) >> 16;
+    const int m0 = be - (int)__umul24(i1, (uint32_t)n0), m1 = (int)(i1 - __umul24(i2, (uint32_t)n1)), m2 = (int)i2;
     const double R = sR[cnt - 1], v = sV[m & 63u];
     if constexpr (PH == 0) {
       tc[j] = (ST)(v * sPh[m0] * sPh[PM_MAXORD + m1] * sPh[2 * PM_MAXORD + m2] * R);
@@ -394,7 +400,7 @@ __global__ void __launch_bounds__(BAL_TPB, PH == 0 ? 6 : 4)
     if (j < total) {
       uint32_t rep;
       int be, cnt;
-      perm_canon<PRUNE, NPW>(A.L, g0, g1, g2, n0, n1, n2, sPt, psh, sPw, T.rcp0, tw[j], rep, be, cnt);
+      perm_canon<PRUNE, NPW>(A.L, g0, g1, g2, n0, n1, n2, sPt, psh, sPw, rcp0, tw[j], rep, be, cnt);
       finish(j, rep, be, cnt);
     }
   }
@@ -682,6 +688,14 @@ static bool perm_build(edk_op *op, PermState *P) {
   PermTab tab;
   memset(&tab, 0, sizeof(tab));
   if (!perm_families(op, tab) || !perm_gens(b, tab)) return false;
+  // reciprocals that split an element number (< 64) into the exponents of
+  // generators 0 and 1, checked exhaustively
+  tab.rcp0 = (65536u + (uint32_t)tab.gen[0].order - 1) / (uint32_t)tab.gen[0].order;
+  tab.rcp1 = (65536u + (uint32_t)tab.gen[1].order - 1) / (uint32_t)tab.gen[1].order;
+  for (uint32_t e = 0; e < PM_MAXGRPORD; e++)
+    if (((e * tab.rcp0) >> 16) != e / (uint32_t)tab.gen[0].order ||
+        ((e * tab.rcp1) >> 16) != e / (uint32_t)tab.gen[1].order)
+      return false;
   // the pruned walk when the basis has its table and it fits in LDS
   P->ptab = (b->d_ptab && b->pbits >= 1 && (1 << b->pbits) <= PM_MAXPT && perm_powers(b->L, tab, P->npw)) ? b->d_ptab
                                                                                                 : nullptr;
